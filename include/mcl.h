@@ -1,0 +1,171 @@
+/*
+ * mcl.h — C ABI of the MI355X trap/charge-kinetics Monte Carlo library (libmcl.so).
+ *
+ * The reference has no FFI: its hot path is the Python step loop in
+ * src/class/simulate.py:46-92 driving TLTrapSim (src/class/tl_trap_lab.py:24-60),
+ * Box (src/class/engine.py:83-188) and Physics (src/class/engine.py:44-77).
+ * Driving that loop one step at a time from Python cannot run at GPU speed, so
+ * the boundary sits one level up: one call advances every (run, sim) cluster
+ * of a sweep through all of its steps.  The Python package mirrors the
+ * reference's classes on top of these entry points (ctypes).
+ *
+ * Conventions
+ *   - Every entry point returns MCL_OK (0) or a negative MCL_E* status; the
+ *     message of the last failure on a context is mcl_last_error(ctx).
+ *   - Plain pointers and sizes only.  Output buffers are caller-owned: host
+ *     memory by default, device memory when MCL_F_DEVICE_PTRS is set.
+ *   - A context owns one HIP stream and its device buffers; it is not
+ *     thread-safe (use one context per host thread).
+ *   - Clusters are ordered run-major then sim, the loop order of
+ *     simulate.py:460-470 and the row order of its CSV (simulate.py:524-532).
+ */
+#ifndef MCL_H
+#define MCL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCL_ABI_VERSION 1
+
+/* status codes (negated errno values) */
+#define MCL_OK         0
+#define MCL_EDEVICE   (-5)   /* HIP runtime error                                  */
+#define MCL_ENOMEM    (-12)  /* device allocation failed                           */
+#define MCL_EINVAL    (-22)  /* invalid argument or parameter set                  */
+#define MCL_ERANGE    (-34)  /* cluster too large for the kernel's LDS budget      */
+#define MCL_EOVERFLOW (-75)  /* output buffer too small for the requested bins     */
+
+/* flags for mcl_run_fixed_dt */
+#define MCL_F_DEVICE_PTRS  0x1u  /* counts / live_steps are device pointers        */
+#define MCL_F_PERSISTENT   0x2u  /* persistent kernel pulling clusters from a queue */
+#define MCL_F_GRAPH        0x4u  /* capture the launch in a hipGraph and replay it  */
+
+/* Philox4x32-10 stream tags: counter word 3 = (sim << 4) | tag */
+#define MCL_TAG_GEOM_E 0u   /* electron coordinates, counter (0, e, run, sim|tag)   */
+#define MCL_TAG_GEOM_H 1u   /* hole coordinates, counter (0, h, run, sim|tag)       */
+#define MCL_TAG_FIRE   2u   /* recombination draw, counter (n>>2, e, run, sim|tag)  */
+#define MCL_TAG_CHAN   3u   /* localisation-channel draw, same counter layout       */
+
+/* Index sentinel for "no live hole" (n_h reached 0). */
+#define MCL_NO_HOLE 0xFFFFu
+
+/*
+ * One parameter set ("run") with its sims = independent clusters.
+ * Field meanings follow the reference YAML keys:
+ *   physics_fp: alpha [1/m], b, s [1/s], E_cb, E_loc_1, E_loc_2 [eV], D0, Retrap,
+ *               k_b [eV/K], D [Gy/s]                       (engine.py:48-60)
+ *   exp_type_fp: rho_prime, holes, boundary_factor, N_e, e_ratio_start,
+ *               T_start [degC], T_rate [K/s]               (tl_trap_lab.py:33-39,
+ *                                                           simulate.py:464-468)
+ * plus the fixed-dt schedule: dt [s] and n_steps.  Temperature at step n is
+ * T_n = (T_start + 273.15) + T_rate * (n * dt), the start-of-step temperature
+ * used by simulate.py:477-479.
+ */
+typedef struct mcl_run_params {
+    double alpha, b, s, E_cb, E_loc_1, E_loc_2, D0, Retrap, k_b, D;
+    double rho_prime, holes, boundary_factor, N_e, e_ratio_start;
+    double T_start, T_rate, dt;
+    int32_t n_steps;
+    int32_t sims;
+} mcl_run_params;
+
+/* Geometry derived from one run (TLTrapSim.__init__, tl_trap_lab.py:33-39;
+ * Box.seed, engine.py:124-129). */
+typedef struct mcl_geometry {
+    double rho;        /* hole density [1/m^3] = rho_prime * 3/(4 pi) * alpha^3      */
+    double side_m;     /* core cube side d = (holes/rho)^(1/3) [m]                   */
+    int32_t n_e0;      /* int(N_e * e_ratio_start) electrons in [0, d)^3             */
+    int32_t n_h;       /* int(holes * bf^3) holes in [0, bf d)^3                     */
+} mcl_geometry;
+
+typedef struct mcl_stats {
+    double   kernel_ms;            /* device time of the MC launch (HIP events)      */
+    uint64_t live_electron_steps;  /* sum over clusters and steps of live electrons  */
+    uint64_t recombinations;       /* total recombination events                     */
+    uint64_t clusters;             /* clusters simulated                             */
+    uint64_t cluster_steps;        /* sum over clusters of simulated steps           */
+} mcl_stats;
+
+/* ---- context ------------------------------------------------------------- */
+int         mcl_abi_version(void);
+int         mcl_create(void** ctx_out, int hip_device);
+void        mcl_destroy(void* ctx);
+const char* mcl_last_error(const void* ctx);
+/* Launch subsequent work on an existing HIP stream (e.g. torch's current
+ * stream); NULL restores the context's own stream. */
+int         mcl_set_stream(void* ctx, void* hip_stream);
+/* Largest cluster (electrons, holes) one workgroup can hold in LDS. */
+int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_holes);
+
+/* ---- geometry (host only) ------------------------------------------------ */
+/* Replaces the geometry part of TLTrapSim.__init__ (tl_trap_lab.py:33-39). */
+int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out);
+
+/* ---- fixed-dt Monte Carlo -------------------------------------------------- */
+/*
+ * Replaces the run x sim x step loop of simulate.py:460-516 (one event per step)
+ * by the fixed-dt multi-recombination step (per live electron one Bernoulli
+ * draw with p = 1 - exp(-k dt), k = s e^{-E_cb/kT} + b e^{-E_loc/kT - alpha r};
+ * conflicts over a shared nearest hole go to the lowest electron index, the
+ * rule of src/est_params/functions.py:228-239).
+ *
+ *   counts     [total_clusters][n_bins] recombinations per bin; bin of step n is
+ *              n * n_bins / n_steps (n_bins <= n_steps of every run; n_bins ==
+ *              n_steps gives per-step counts, the reference's Lum column).
+ *   live_steps [total_clusters] live electron-steps per cluster, or NULL.
+ * The Dose filling path (D > 0) is not part of the fixed-dt kernel: runs with
+ * D != 0 are rejected with MCL_EINVAL, matching simulate.py:466 which always
+ * runs with dose_rate 0.
+ */
+int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs,
+                     uint64_t seed, int32_t n_bins, uint32_t flags,
+                     uint32_t* counts, uint64_t* live_steps, mcl_stats* stats);
+
+/*
+ * Plans: upload the parameter sets once, launch many times.  A plan owns the
+ * device-side run descriptors and cluster map (and, with MCL_F_GRAPH, a
+ * captured hipGraph).  mcl_plan_launch is asynchronous on the context's
+ * stream and takes DEVICE pointers for counts [total_clusters][n_bins] and
+ * live_steps [total_clusters] (may be NULL); mcl_sync waits for it.
+ */
+typedef struct mcl_plan_info {
+    int64_t total_clusters;
+    int32_t n_bins;
+    int32_t lds_bytes;        /* dynamic LDS per workgroup                       */
+    int32_t max_electrons;    /* largest n_e0 in the plan                        */
+    int32_t max_holes;        /* largest n_h in the plan                         */
+    int32_t grid;             /* workgroups launched                             */
+    int32_t block;            /* threads per workgroup                           */
+} mcl_plan_info;
+
+int  mcl_plan_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs,
+                       uint64_t seed, int32_t n_bins, uint32_t flags, void** plan_out);
+int  mcl_plan_info_get(const void* plan, mcl_plan_info* info);
+int  mcl_plan_launch(void* plan, uint32_t* counts_dev, uint64_t* live_steps_dev);
+/* Device time of the last launch of this plan (HIP events on the launch stream). */
+int  mcl_plan_last_ms(void* plan, double* ms);
+void mcl_plan_destroy(void* plan);
+int  mcl_sync(void* ctx);
+
+/*
+ * State of cluster (run_index, sim) as the kernel holds it after max_steps
+ * steps (0 = the initial state): coordinates [nm] in float32, which electrons
+ * and holes are left, the incrementally maintained nearest live hole per
+ * electron (argmin, first index on ties) and w = exp(-alpha r).  Replaces
+ * Box.seed, Box._rebuild and Box.remove_pair's bookkeeping (engine.py:113-175)
+ * for inspection.  Host pointers: e_xyz [n_e0][3], h_xyz [n_h][3] (a removed
+ * hole's x reads 3e38), e_live [n_e0], h_alive [n_h], nn [n_e0], w [n_e0].
+ */
+int mcl_cluster_state(void* ctx, const mcl_run_params* run, int32_t run_index,
+                      int32_t sim, uint64_t seed, int32_t max_steps,
+                      float* e_xyz, float* h_xyz, uint8_t* e_live, uint8_t* h_alive,
+                      uint32_t* nn, float* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCL_H */

@@ -1,0 +1,272 @@
+"""ctypes binding of libmcl.so, the C ABI declared in include/mcl.h.
+
+This is the only way the package reaches the GPU.  There is no CPU fallback:
+if the library is missing or no HIP device is present, ``Library()`` and
+``Context()`` raise, naming what is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libmcl.so"
+
+ABI_VERSION = 1
+
+MCL_OK = 0
+MCL_EDEVICE = -5
+MCL_ENOMEM = -12
+MCL_EINVAL = -22
+MCL_ERANGE = -34
+MCL_EOVERFLOW = -75
+
+F_DEVICE_PTRS = 0x1
+F_PERSISTENT = 0x2
+F_GRAPH = 0x4
+
+TAG_GEOM_E = 0
+TAG_GEOM_H = 1
+TAG_FIRE = 2
+TAG_CHAN = 3
+NO_HOLE = 0xFFFF
+
+DEFAULT_SEED = 0x4D434C554D000001
+
+
+class RunParams(C.Structure):
+    """mcl_run_params: one parameter set (run) and its number of clusters (sims)."""
+
+    _fields_ = [
+        ("alpha", C.c_double), ("b", C.c_double), ("s", C.c_double), ("E_cb", C.c_double),
+        ("E_loc_1", C.c_double), ("E_loc_2", C.c_double), ("D0", C.c_double),
+        ("Retrap", C.c_double), ("k_b", C.c_double), ("D", C.c_double),
+        ("rho_prime", C.c_double), ("holes", C.c_double), ("boundary_factor", C.c_double),
+        ("N_e", C.c_double), ("e_ratio_start", C.c_double),
+        ("T_start", C.c_double), ("T_rate", C.c_double), ("dt", C.c_double),
+        ("n_steps", C.c_int32), ("sims", C.c_int32),
+    ]
+
+    FLOAT_FIELDS = ("alpha", "b", "s", "E_cb", "E_loc_1", "E_loc_2", "D0", "Retrap", "k_b", "D",
+                    "rho_prime", "holes", "boundary_factor", "N_e", "e_ratio_start",
+                    "T_start", "T_rate", "dt")
+
+    @classmethod
+    def make(cls, **kw) -> "RunParams":
+        p = cls()
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError(f"unknown run parameter {k!r}")
+            setattr(p, k, v)
+        return p
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Geometry(C.Structure):
+    _fields_ = [("rho", C.c_double), ("side_m", C.c_double), ("n_e0", C.c_int32),
+                ("n_h", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("live_electron_steps", C.c_uint64),
+                ("recombinations", C.c_uint64), ("clusters", C.c_uint64),
+                ("cluster_steps", C.c_uint64)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("total_clusters", C.c_int64), ("n_bins", C.c_int32), ("lds_bytes", C.c_int32),
+                ("max_electrons", C.c_int32), ("max_holes", C.c_int32), ("grid", C.c_int32),
+                ("block", C.c_int32)]
+
+
+class MclError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mcl error {code}: {msg}")
+        self.code = code
+
+
+_P = C.c_void_p
+_SIGNATURES = {
+    "mcl_abi_version": (C.c_int, []),
+    "mcl_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "mcl_destroy": (None, [_P]),
+    "mcl_last_error": (C.c_char_p, [_P]),
+    "mcl_set_stream": (C.c_int, [_P, _P]),
+    "mcl_cluster_limits": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mcl_run_geometry": (C.c_int, [C.POINTER(RunParams), C.POINTER(Geometry)]),
+    "mcl_run_fixed_dt": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_uint64, C.c_int32,
+                                   C.c_uint32, _P, _P, C.POINTER(Stats)]),
+    "mcl_plan_fixed_dt": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_uint64, C.c_int32,
+                                    C.c_uint32, C.POINTER(_P)]),
+    "mcl_plan_info_get": (C.c_int, [_P, C.POINTER(PlanInfo)]),
+    "mcl_plan_launch": (C.c_int, [_P, _P, _P]),
+    "mcl_plan_last_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mcl_plan_destroy": (None, [_P]),
+    "mcl_sync": (C.c_int, [_P]),
+    "mcl_cluster_state": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
+                                    C.c_int32, _P, _P, _P, _P, _P, _P]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib: Optional[C.CDLL] = None
+
+
+def library() -> C.CDLL:
+    """Load libmcl.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -m mcluminescence_amd.csrc.build); there is no CPU fallback")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.mcl_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"libmcl ABI {lib.mcl_abi_version()} != expected {ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def run_array(runs: Sequence[RunParams]):
+    arr = (RunParams * len(runs))()
+    for i, r in enumerate(runs):
+        arr[i] = r
+    return arr
+
+
+def geometry(run: RunParams) -> Geometry:
+    g = Geometry()
+    st = library().mcl_run_geometry(C.byref(run), C.byref(g))
+    if st != MCL_OK:
+        raise MclError(st, "mcl_run_geometry")
+    return g
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Context:
+    """A libmcl context: one HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = library()
+        h = C.c_void_p()
+        st = self.lib.mcl_create(C.byref(h), int(device))
+        if st != MCL_OK:
+            raise MclError(st, f"mcl_create(device={device}) failed: no usable HIP device")
+        self.handle = h
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.mcl_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int, what: str) -> None:
+        if st != MCL_OK:
+            msg = self.lib.mcl_last_error(self.handle)
+            raise MclError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_stream(self, stream_handle: Optional[int]) -> None:
+        self.check(self.lib.mcl_set_stream(self.handle, stream_handle or None), "mcl_set_stream")
+
+    def cluster_limits(self):
+        e, h = C.c_int32(), C.c_int32()
+        self.check(self.lib.mcl_cluster_limits(self.handle, C.byref(e), C.byref(h)),
+                   "mcl_cluster_limits")
+        return e.value, h.value
+
+    def run_fixed_dt(self, runs: Sequence[RunParams], seed: int = DEFAULT_SEED,
+                     n_bins: Optional[int] = None):
+        """Run every cluster of every run; returns (counts [clusters, n_bins] uint32,
+        live_steps [clusters] uint64, Stats)."""
+        arr = run_array(runs)
+        if n_bins is None:
+            n_bins = min(int(r.n_steps) for r in runs)
+        total = sum(int(r.sims) for r in runs)
+        counts = np.zeros((total, n_bins), dtype=np.uint32)
+        live = np.zeros(total, dtype=np.uint64)
+        stats = Stats()
+        st = self.lib.mcl_run_fixed_dt(self.handle, arr, len(runs), C.c_uint64(seed), int(n_bins),
+                                       0, _ptr(counts), _ptr(live), C.byref(stats))
+        self.check(st, "mcl_run_fixed_dt")
+        return counts, live, stats
+
+    def cluster_state(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
+                      max_steps: int = 0) -> dict:
+        g = geometry(run)
+        ne, nh = max(g.n_e0, 0), max(g.n_h, 0)
+        e_xyz = np.zeros((ne, 3), np.float32)
+        h_xyz = np.zeros((nh, 3), np.float32)
+        e_live = np.zeros(ne, np.uint8)
+        h_alive = np.zeros(nh, np.uint8)
+        nn = np.zeros(ne, np.uint32)
+        w = np.zeros(ne, np.float32)
+        st = self.lib.mcl_cluster_state(self.handle, C.byref(run), int(run_index), int(sim),
+                                        C.c_uint64(seed), int(max_steps), _ptr(e_xyz), _ptr(h_xyz),
+                                        _ptr(e_live), _ptr(h_alive), _ptr(nn), _ptr(w))
+        self.check(st, "mcl_cluster_state")
+        return dict(e_xyz=e_xyz, h_xyz=h_xyz, e_live=e_live, h_alive=h_alive, nn=nn, w=w)
+
+    def plan(self, runs: Sequence[RunParams], seed: int = DEFAULT_SEED,
+             n_bins: Optional[int] = None, flags: int = 0) -> "Plan":
+        return Plan(self, runs, seed, n_bins, flags)
+
+
+class Plan:
+    """Device-resident parameter sets, launched many times (mcl_plan_*)."""
+
+    def __init__(self, ctx: Context, runs: Sequence[RunParams], seed: int,
+                 n_bins: Optional[int], flags: int):
+        self.ctx = ctx
+        arr = run_array(runs)
+        if n_bins is None:
+            n_bins = min(int(r.n_steps) for r in runs)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.mcl_plan_fixed_dt(ctx.handle, arr, len(runs), C.c_uint64(seed),
+                                            int(n_bins), int(flags), C.byref(h)),
+                  "mcl_plan_fixed_dt")
+        self.handle = h
+        info = PlanInfo()
+        ctx.check(ctx.lib.mcl_plan_info_get(h, C.byref(info)), "mcl_plan_info_get")
+        self.info = info
+
+    def launch(self, counts_dev_ptr: int, live_dev_ptr: Optional[int] = None) -> None:
+        self.ctx.check(self.ctx.lib.mcl_plan_launch(self.handle, counts_dev_ptr, live_dev_ptr),
+                       "mcl_plan_launch")
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.mcl_plan_last_ms(self.handle, C.byref(ms)), "mcl_plan_last_ms")
+        return ms.value
+
+    def sync(self) -> None:
+        self.ctx.check(self.ctx.lib.mcl_sync(self.ctx.handle), "mcl_sync")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.ctx.lib.mcl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

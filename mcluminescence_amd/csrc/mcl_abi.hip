@@ -1,0 +1,507 @@
+// mcl_abi.hip — the C ABI of libmcl.so (declared in include/mcl.h).
+//
+// Host side of the drop-in boundary: validates parameter sets the way the
+// reference's classes would fail on them, derives each run's geometry
+// (tl_trap_lab.py:33-39, engine.py:124-129), uploads descriptors once per plan
+// and launches the HIP kernels on the context's stream.  No CPU fallback: if
+// no HIP device is usable every entry point fails with MCL_EDEVICE.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mcl.h"
+#include "mcl_internal.h"
+
+namespace {
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int max_lds = 0;        // dynamic LDS bytes one workgroup may use
+    int n_cu = 0;
+    std::string err;
+};
+
+struct Plan {
+    Ctx* ctx = nullptr;
+    mcl::RunDesc* d_runs = nullptr;
+    int32_t* d_crun = nullptr;
+    int32_t* d_csim = nullptr;
+    uint32_t key0 = 0, key1 = 0;
+    int64_t total = 0;
+    int32_t n_bins = 0;
+    int32_t h_cap = 0, e_cap = 0, lds = 0;
+    int32_t max_e = 0, max_h = 0;
+    uint32_t flags = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+int fail(Ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                    \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "%s: %s", \
+                        #expr, hipGetErrorString(e_));                                        \
+    } while (0)
+
+constexpr double kPi = 0x1.921fb54442d18p+1;
+
+void geometry(const mcl_run_params& p, mcl_geometry& g)
+{
+    // tl_trap_lab.py:33-34 and engine.py:127 (int() truncation)
+    g.rho = p.rho_prime * (3.0 / (4.0 * kPi) * std::pow(p.alpha, 3.0));
+    g.side_m = std::pow(p.holes / g.rho, 1.0 / 3.0);
+    g.n_e0 = (int32_t)(p.N_e * p.e_ratio_start);
+    g.n_h = (int32_t)(p.holes * std::pow(p.boundary_factor, 3.0));
+}
+
+bool finite_all(const mcl_run_params& p)
+{
+    const double v[] = {p.alpha, p.b, p.s, p.E_cb, p.E_loc_1, p.E_loc_2, p.D0, p.Retrap, p.k_b, p.D,
+                        p.rho_prime, p.holes, p.boundary_factor, p.N_e, p.e_ratio_start,
+                        p.T_start, p.T_rate, p.dt};
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geometry& g)
+{
+    if (!finite_all(p)) return fail(c, MCL_EINVAL, "run %d: non-finite parameter", r);
+    if (!(p.dt > 0.0)) return fail(c, MCL_EINVAL, "run %d: dt must be > 0 (got %g)", r, p.dt);
+    if (p.n_steps <= 0) return fail(c, MCL_EINVAL, "run %d: n_steps must be > 0", r);
+    if (p.sims < 0) return fail(c, MCL_EINVAL, "run %d: sims must be >= 0", r);
+    if (p.D != 0.0)
+        return fail(c, MCL_EINVAL,
+                    "run %d: dose filling (D=%g) is not part of the fixed-dt kernel", r, p.D);
+    if (!(p.k_b > 0.0)) return fail(c, MCL_EINVAL, "run %d: k_b must be > 0", r);
+    if (!(p.alpha > 0.0) || !(p.rho_prime > 0.0) || !(p.holes > 0.0) || !(p.boundary_factor > 0.0))
+        return fail(c, MCL_EINVAL, "run %d: alpha, rho_prime, holes, boundary_factor must be > 0", r);
+    if (p.s < 0.0 || p.b < 0.0) return fail(c, MCL_EINVAL, "run %d: s and b must be >= 0", r);
+    const double T0 = p.T_start + 273.15;
+    const double T1 = T0 + p.T_rate * ((double)(p.n_steps - 1) * p.dt);
+    if (!(T0 > 0.0) || !(T1 > 0.0))
+        return fail(c, MCL_EINVAL, "run %d: temperature must stay > 0 K (%g .. %g K)", r, T0, T1);
+    if (n_bins <= 0 || n_bins > p.n_steps)
+        return fail(c, MCL_EOVERFLOW, "run %d: n_bins=%d must be in [1, n_steps=%d]", r, n_bins,
+                    p.n_steps);
+    geometry(p, g);
+    if (g.n_e0 < 0 || g.n_h < 0) return fail(c, MCL_EINVAL, "run %d: negative particle count", r);
+    if (g.n_e0 > 65535 || g.n_h > 65534)
+        return fail(c, MCL_ERANGE, "run %d: %d electrons / %d holes exceed 16-bit indices", r,
+                    g.n_e0, g.n_h);
+    return MCL_OK;
+}
+
+mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id)
+{
+    mcl::RunDesc d{};
+    d.side_nm = g.side_m * 1e9;
+    d.side_b_nm = (g.side_m * p.boundary_factor) * 1e9;
+    d.alpha_nm = p.alpha * 1e-9;
+    d.s = p.s;
+    d.b = p.b;
+    d.E_cb = p.E_cb;
+    d.E_loc_1 = p.E_loc_1;
+    d.E_loc_2 = p.E_loc_2;
+    d.k_b = p.k_b;
+    d.T0_K = p.T_start + 273.15;
+    d.T_rate = p.T_rate;
+    d.dt = p.dt;
+    d.two_channel = (p.E_loc_1 != p.E_loc_2) && (p.Retrap > 0.0) && (p.Retrap < 1.0);
+    d.always_chan2 = p.Retrap >= 1.0;
+    d.ret_thr = d.two_channel ? (uint64_t)(p.Retrap * 0x1p32) : 0u;
+    d.n_e0 = g.n_e0;
+    d.n_h = g.n_h;
+    d.n_steps = p.n_steps;
+    d.run_id = run_id;
+    return d;
+}
+
+void plan_free(Plan* p)
+{
+    if (!p) return;
+    hipFree(p->d_runs);
+    hipFree(p->d_crun);
+    hipFree(p->d_csim);
+    if (p->ev0) hipEventDestroy(p->ev0);
+    if (p->ev1) hipEventDestroy(p->ev1);
+    delete p;
+}
+
+// Build a plan; run_ids overrides the Philox run index of each run (may be null).
+int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t* run_ids,
+              uint64_t seed, int32_t n_bins, uint32_t flags, Plan** out)
+{
+    if (!runs || n_runs <= 0) return fail(c, MCL_EINVAL, "no runs");
+    std::vector<mcl::RunDesc> desc((size_t)n_runs);
+    int64_t total = 0;
+    int max_e = 0, max_h = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        mcl_geometry g;
+        int st = validate_run(c, runs[r], r, n_bins, g);
+        if (st) return st;
+        desc[(size_t)r] = make_desc(runs[r], g, run_ids ? run_ids[r] : r);
+        total += runs[r].sims;
+        if (runs[r].sims > 0) {
+            if (g.n_e0 > max_e) max_e = g.n_e0;
+            if (g.n_h > max_h) max_h = g.n_h;
+        }
+    }
+    if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
+    if (total > 0x7FFFFFFF) return fail(c, MCL_EINVAL, "too many clusters (%lld)", (long long)total);
+    const int h_cap = ((max_h + 3) / 4) * 4 + 4;
+    const int e_cap = ((max_e + 63) / 64) * 64 + 64;
+    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap);
+    if (lds > c->max_lds)
+        return fail(c, MCL_ERANGE,
+                    "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B",
+                    max_e, max_h, lds, c->max_lds);
+    std::vector<int32_t> crun((size_t)total), csim((size_t)total);
+    int64_t k = 0;
+    for (int r = 0; r < n_runs; ++r)
+        for (int s = 0; s < runs[r].sims; ++s, ++k) {
+            crun[(size_t)k] = r;
+            csim[(size_t)k] = s;
+        }
+    Plan* p = new (std::nothrow) Plan();
+    if (!p) return fail(c, MCL_ENOMEM, "host allocation failed");
+    p->ctx = c;
+    p->key0 = (uint32_t)seed;
+    p->key1 = (uint32_t)(seed >> 32);
+    p->total = total;
+    p->n_bins = n_bins;
+    p->h_cap = h_cap;
+    p->e_cap = e_cap;
+    p->lds = lds;
+    p->max_e = max_e;
+    p->max_h = max_h;
+    p->flags = flags;
+    hipError_t e = hipMalloc((void**)&p->d_runs, sizeof(mcl::RunDesc) * (size_t)n_runs);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_crun, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_csim, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_runs, desc.data(), sizeof(mcl::RunDesc) * (size_t)n_runs,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_crun, crun.data(), sizeof(int32_t) * (size_t)total,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_csim, csim.data(), sizeof(int32_t) * (size_t)total,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev1);
+    if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "plan setup: %s",
+                    hipGetErrorString(e));
+    }
+    *out = p;
+    return MCL_OK;
+}
+
+mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
+{
+    mcl::LaunchArgs a{};
+    a.runs = p->d_runs;
+    a.c_run = p->d_crun;
+    a.c_sim = p->d_csim;
+    a.n_clusters = (int32_t)p->total;
+    a.key0 = p->key0;
+    a.key1 = p->key1;
+    a.n_bins = p->n_bins;
+    a.counts = counts;
+    a.live_steps = live;
+    a.h_cap = p->h_cap;
+    a.e_cap = p->e_cap;
+    a.dump.enabled = 0;
+    return a;
+}
+
+int plan_launch(Plan* p, uint32_t* counts, uint64_t* live, const mcl::StateDump* dump)
+{
+    Ctx* c = p->ctx;
+    if (!counts) return fail(c, MCL_EINVAL, "counts is null");
+    mcl::LaunchArgs a = launch_args(p, counts, live);
+    if (dump) a.dump = *dump;
+    HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
+    HIP_TRY(c, mcl::launch_fixed_dt(a, p->lds, c->stream));
+    HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
+    p->timed = true;
+    return MCL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcl_abi_version(void) { return MCL_ABI_VERSION; }
+
+int mcl_create(void** ctx_out, int hip_device)
+{
+    if (!ctx_out) return MCL_EINVAL;
+    *ctx_out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MCL_EDEVICE;
+    if (hip_device < 0 || hip_device >= n) return MCL_EINVAL;
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) return MCL_ENOMEM;
+    c->device = hip_device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(hip_device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, hip_device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MCL_EDEVICE;
+    }
+    c->stream = c->own_stream;
+    c->n_cu = prop.multiProcessorCount;
+    c->max_lds = (int)prop.sharedMemPerBlock;
+    *ctx_out = c;
+    return MCL_OK;
+}
+
+void mcl_destroy(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return;
+    if (c->own_stream) {
+        hipStreamSynchronize(c->own_stream);
+        hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+}
+
+const char* mcl_last_error(const void* ctx)
+{
+    const Ctx* c = (const Ctx*)ctx;
+    return c ? c->err.c_str() : "null context";
+}
+
+int mcl_set_stream(void* ctx, void* hip_stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    c->stream = hip_stream ? (hipStream_t)hip_stream : c->own_stream;
+    return MCL_OK;
+}
+
+int mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_holes)
+{
+    const Ctx* c = (const Ctx*)ctx;
+    if (!c || !max_electrons || !max_holes) return MCL_EINVAL;
+    // equal electron and hole counts: 16 (h + 4) + 20 (e + 64) <= max_lds
+    int n = (c->max_lds - 16 * 4 - 20 * 64) / 36;
+    n = (n / 64) * 64;
+    *max_electrons = n > 65535 ? 65535 : n;
+    *max_holes = n > 65534 ? 65534 : n;
+    return MCL_OK;
+}
+
+int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out)
+{
+    if (!run || !out) return MCL_EINVAL;
+    geometry(*run, *out);
+    return MCL_OK;
+}
+
+int mcl_plan_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint64_t seed,
+                      int32_t n_bins, uint32_t flags, void** plan_out)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !plan_out) return MCL_EINVAL;
+    *plan_out = nullptr;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    Plan* p = nullptr;
+    int st = make_plan(c, runs, n_runs, nullptr, seed, n_bins, flags, &p);
+    if (st) return st;
+    *plan_out = p;
+    return MCL_OK;
+}
+
+int mcl_plan_info_get(const void* plan, mcl_plan_info* info)
+{
+    const Plan* p = (const Plan*)plan;
+    if (!p || !info) return MCL_EINVAL;
+    info->total_clusters = p->total;
+    info->n_bins = p->n_bins;
+    info->lds_bytes = p->lds;
+    info->max_electrons = p->max_e;
+    info->max_holes = p->max_h;
+    info->grid = (int32_t)p->total;
+    info->block = mcl::kWaveHost;
+    return MCL_OK;
+}
+
+int mcl_plan_launch(void* plan, uint32_t* counts_dev, uint64_t* live_steps_dev)
+{
+    Plan* p = (Plan*)plan;
+    if (!p) return MCL_EINVAL;
+    return plan_launch(p, counts_dev, live_steps_dev, nullptr);
+}
+
+int mcl_plan_last_ms(void* plan, double* ms)
+{
+    Plan* p = (Plan*)plan;
+    if (!p || !ms) return MCL_EINVAL;
+    if (!p->timed) return fail(p->ctx, MCL_EINVAL, "plan has not been launched");
+    HIP_TRY(p->ctx, hipEventSynchronize(p->ev1));
+    float f = 0.0f;
+    HIP_TRY(p->ctx, hipEventElapsedTime(&f, p->ev0, p->ev1));
+    *ms = (double)f;
+    return MCL_OK;
+}
+
+void mcl_plan_destroy(void* plan) { plan_free((Plan*)plan); }
+
+int mcl_sync(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MCL_OK;
+}
+
+int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint64_t seed,
+                     int32_t n_bins, uint32_t flags, uint32_t* counts, uint64_t* live_steps,
+                     mcl_stats* stats)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!counts) return fail(c, MCL_EINVAL, "counts is null");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    Plan* p = nullptr;
+    int st = make_plan(c, runs, n_runs, nullptr, seed, n_bins, flags, &p);
+    if (st) return st;
+    const bool dev = (flags & MCL_F_DEVICE_PTRS) != 0;
+    const size_t n_counts = (size_t)p->total * (size_t)n_bins;
+    uint32_t* d_counts = dev ? counts : nullptr;
+    uint64_t* d_live = nullptr;
+    hipError_t e = hipSuccess;
+    if (!dev) e = hipMalloc((void**)&d_counts, sizeof(uint32_t) * n_counts);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_live, sizeof(uint64_t) * (size_t)p->total);
+    if (e != hipSuccess) {
+        if (!dev) hipFree(d_counts);
+        plan_free(p);
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "output allocation: %s",
+                    hipGetErrorString(e));
+    }
+    st = plan_launch(p, d_counts, d_live, nullptr);
+    std::vector<uint64_t> live((size_t)p->total);
+    if (st == MCL_OK) {
+        e = hipMemcpyAsync(live.data(), d_live, sizeof(uint64_t) * (size_t)p->total,
+                           hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && !dev)
+            e = hipMemcpyAsync(counts, d_counts, sizeof(uint32_t) * n_counts, hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess && live_steps)
+            e = dev ? hipMemcpyAsync(live_steps, d_live, sizeof(uint64_t) * (size_t)p->total,
+                                     hipMemcpyDeviceToDevice, c->stream)
+                    : hipSuccess;
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+            st = fail(c, MCL_EDEVICE, "kernel or copy failed: %s", hipGetErrorString(e));
+    }
+    if (st == MCL_OK && live_steps && !dev)
+        std::memcpy(live_steps, live.data(), sizeof(uint64_t) * (size_t)p->total);
+    if (st == MCL_OK && stats) {
+        std::memset(stats, 0, sizeof *stats);
+        double ms = 0.0;
+        mcl_plan_last_ms(p, &ms);
+        stats->kernel_ms = ms;
+        stats->clusters = (uint64_t)p->total;
+        for (uint64_t v : live) stats->live_electron_steps += v;
+        if (!dev) {
+            for (size_t i = 0; i < n_counts; ++i) stats->recombinations += counts[i];
+        }
+        int64_t k = 0;
+        for (int r = 0; r < n_runs; ++r)
+            for (int s = 0; s < runs[r].sims; ++s, ++k) stats->cluster_steps += (uint64_t)runs[r].n_steps;
+    }
+    if (!dev) hipFree(d_counts);
+    hipFree(d_live);
+    plan_free(p);
+    return st;
+}
+
+int mcl_cluster_state(void* ctx, const mcl_run_params* run, int32_t run_index, int32_t sim,
+                      uint64_t seed, int32_t max_steps, float* e_xyz, float* h_xyz,
+                      uint8_t* e_live, uint8_t* h_alive, uint32_t* nn, float* w)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!run || !e_xyz || !h_xyz || !e_live || !h_alive || !nn || !w)
+        return fail(c, MCL_EINVAL, "null output pointer");
+    if (run_index < 0 || sim < 0 || max_steps < 0)
+        return fail(c, MCL_EINVAL, "run_index, sim and max_steps must be >= 0");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    mcl_run_params one = *run;
+    one.sims = sim + 1;   // clusters 0..sim; only the last one is dumped
+    Plan* p = nullptr;
+    int st = make_plan(c, &one, 1, &run_index, seed, 1, 0, &p);
+    if (st) return st;
+    mcl_geometry g;
+    geometry(one, g);
+    const size_t ne = (size_t)(g.n_e0 > 0 ? g.n_e0 : 0), nh = (size_t)(g.n_h > 0 ? g.n_h : 0);
+    // only the requested sim is simulated: a one-cluster map pointing at it
+    const int32_t zero = 0;
+    hipError_t e = hipMemcpyAsync(p->d_crun, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_csim, &sim, sizeof sim, hipMemcpyHostToDevice, c->stream);
+    p->total = 1;
+    char* buf = nullptr;
+    const size_t bytes = ne * 12 + nh * 12 + ne + nh + ne * 4 + ne * 4 + 64 + 64;
+    if (e == hipSuccess) e = hipMalloc((void**)&buf, bytes);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return fail(c, MCL_EDEVICE, "cluster_state setup: %s", hipGetErrorString(e));
+    }
+    mcl::StateDump d{};
+    size_t off = 0;
+    d.e_xyz = (float*)(buf + off); off += ne * 12;
+    d.h_xyz = (float*)(buf + off); off += nh * 12;
+    d.nn = (uint32_t*)(buf + off); off += ne * 4;
+    d.w = (float*)(buf + off); off += ne * 4;
+    d.e_live = (uint8_t*)(buf + off); off += ne;
+    d.h_alive = (uint8_t*)(buf + off); off += nh;
+    uint32_t* d_counts = (uint32_t*)(buf + off + (64 - (off % 64)) % 64);
+    d.max_steps = max_steps;
+    d.enabled = 1;
+    st = plan_launch(p, d_counts, nullptr, &d);
+    if (st == MCL_OK) {
+        e = hipMemcpyAsync(e_xyz, d.e_xyz, ne * 12, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_xyz, d.h_xyz, nh * 12, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(nn, d.nn, ne * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(w, d.w, ne * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(e_live, d.e_live, ne, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_alive, d.h_alive, nh, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = fail(c, MCL_EDEVICE, "cluster_state: %s", hipGetErrorString(e));
+    }
+    hipFree(buf);
+    plan_free(p);
+    return st;
+}
+
+}  // extern "C"

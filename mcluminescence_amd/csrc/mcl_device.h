@@ -1,0 +1,196 @@
+// mcl_device.h — device building blocks for the CDNA4 Monte Carlo kernels:
+// Philox4x32-10 counter RNG, the library's own float32/float64 exp and expm1,
+// the per-electron fire threshold, and wave64 reductions.
+//
+// Every float operation that feeds a recombination decision is written with an
+// explicit rounding: the library is compiled with -ffp-contract=off, products
+// that should fuse call __builtin_fma{,f}, and exp/expm1 are evaluated here
+// (Cody-Waite reduction with ln2 split 12+24 bits in float32 / 21+53 bits in
+// float64, Taylor polynomial in Horner form) rather than by the hardware
+// v_exp_f32, whose last-bit behaviour no CPU reproduces.  A CPU evaluation of
+// the same steps in the same order gives the same bits, which is what makes
+// per-step replay bit-exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcl {
+
+constexpr int kWave = 64;
+constexpr uint32_t kNoHole = 0xFFFFu;
+constexpr uint32_t kDeadSlot = 0xFFFFFFFFu;
+constexpr float kDeadHoleX = 3.0e38f;   // removed hole: x coordinate sentinel, d2 -> +inf
+
+// ------------------------------------------------------------------ Philox --
+constexpr uint32_t kPhM0 = 0xD2511F53u, kPhM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhW0 = 0x9E3779B9u, kPhW1 = 0xBB67AE85u;
+
+struct u32x4 { uint32_t x, y, z, w; };
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The key is wave-uniform,
+// so its bumps stay on the scalar unit; each round is two 32x32->64 multiplies
+// (v_mad_u64_u32) and two three-way xors.
+__device__ __forceinline__ u32x4 philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                          uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        if (i) { k0 += kPhW0; k1 += kPhW1; }
+        const uint64_t p0 = (uint64_t)kPhM0 * c0;
+        const uint64_t p1 = (uint64_t)kPhM1 * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ uint32_t pick_word(const u32x4& r, int word)
+{
+    return word == 0 ? r.x : word == 1 ? r.y : word == 2 ? r.z : r.w;
+}
+
+// ------------------------------------------------------- own exp / expm1 ----
+constexpr float kFInvLn2 = 0x1.715476p+0f;
+constexpr float kFLn2Hi = 0x1.62ep-1f;
+constexpr float kFLn2Lo = 0x1.0bfbe8p-15f;
+
+__device__ __forceinline__ float pow2i_f(int k)   // 2^k, -126 <= k <= 127
+{
+    return __int_as_float((k + 127) << 23);
+}
+
+// expm1(r) on |r| <= ln2/2: r + r^2 (1/2! + r/3! + ... + r^6/8!)
+__device__ __forceinline__ float em1_core(float r)
+{
+    float p = 0x1.a01a02p-16f;
+    p = __builtin_fmaf(p, r, 0x1.a01a02p-13f);
+    p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+    p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+    p = __builtin_fmaf(p, r, 0x1p-1f);
+    const float r2 = r * r;
+    return __builtin_fmaf(r2, p, r);
+}
+
+__device__ __forceinline__ float expm1_own(float x)
+{
+    if (x != x) return x;
+    if (x < -17.5f) return -1.0f;
+    if (x > 88.0f) return __builtin_inff();
+    const float kf = __builtin_rintf(x * kFInvLn2);
+    float r = __builtin_fmaf(-kf, kFLn2Hi, x);
+    r = __builtin_fmaf(-kf, kFLn2Lo, r);
+    const float em1 = em1_core(r);
+    if (kf == 0.0f) return em1;
+    const float t = pow2i_f((int)kf);
+    return __builtin_fmaf(t, em1, t - 1.0f);
+}
+
+__device__ __forceinline__ float exp_own(float x)
+{
+    if (x != x) return x;
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return __builtin_inff();
+    const float kf = __builtin_rintf(x * kFInvLn2);
+    float r = __builtin_fmaf(-kf, kFLn2Hi, x);
+    r = __builtin_fmaf(-kf, kFLn2Lo, r);
+    const float e = 1.0f + em1_core(r);
+    return e * pow2i_f((int)kf);
+}
+
+constexpr double kDInvLn2 = 0x1.71547652b82fep+0;
+constexpr double kDLn2Hi = 0x1.62e42p-1;
+constexpr double kDLn2Lo = 0x1.fdf473de6af28p-22;
+
+__device__ __forceinline__ double pow2i_d(int k)  // 2^k, -1022 <= k <= 1023
+{
+    return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+__device__ __forceinline__ double exp_own(double x)
+{
+    if (x != x) return x;
+    if (x < -708.0) return 0.0;
+    if (x > 709.0) return __builtin_inf();
+    const double kd = __builtin_rint(x * kDInvLn2);
+    double r = __builtin_fma(-kd, kDLn2Hi, x);
+    r = __builtin_fma(-kd, kDLn2Lo, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0x1p-1);
+    p = __builtin_fma(p, r, 0x1p+0);
+    const double e = __builtin_fma(p, r, 1.0);
+    return e * pow2i_d((int)kd);
+}
+
+// Fire threshold for x = k dt: p = 1 - e^{-x}; an electron with draw u fires
+// iff always || u < thr, thr = (uint32)(p * 2^32).
+struct FireThr { uint32_t thr; bool always; };
+
+__device__ __forceinline__ FireThr fire_threshold(float x)
+{
+    const float p = -expm1_own(-x);
+    FireThr t;
+    t.always = p >= 1.0f;
+    t.thr = t.always ? 0xFFFFFFFFu : (uint32_t)(p * 0x1p32f);
+    return t;
+}
+
+// ---------------------------------------------------------- wave helpers ----
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// exclusive prefix count of set bits of mask below this lane
+__device__ __forceinline__ int lanes_below(uint64_t mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), off);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), off);
+        v |= ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+}  // namespace mcl

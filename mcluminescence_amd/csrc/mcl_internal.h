@@ -1,0 +1,63 @@
+// mcl_internal.h — host/device shared descriptors of libmcl (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcl {
+
+// Per-run constants of the fixed-dt kernel, derived on the host from
+// mcl_run_params (geometry of tl_trap_lab.py:33-39 and engine.py:124-129).
+struct RunDesc {
+    double side_nm;     // core cube side [nm]
+    double side_b_nm;   // hole cube side bf * d [nm]
+    double alpha_nm;    // alpha [1/nm]
+    double s, b, E_cb, E_loc_1, E_loc_2, k_b;
+    double T0_K;        // T_start + 273.15
+    double T_rate;      // K/s
+    double dt;          // s
+    uint64_t ret_thr;   // floor(Retrap * 2^32): channel 2 iff u < ret_thr
+    int32_t n_e0, n_h, n_steps;
+    int32_t two_channel;   // draw the localisation channel per electron-step
+    int32_t always_chan2;  // Retrap >= 1: every draw picks E_loc_2
+    int32_t run_id;        // Philox counter word 2 (the run's index in its sweep)
+};
+
+// Optional dump of one cluster's kernel state (tests / inspection).
+struct StateDump {
+    float* e_xyz;       // [n_e0][3]
+    float* h_xyz;       // [n_h][3]
+    uint8_t* e_live;    // [n_e0]
+    uint8_t* h_alive;   // [n_h]
+    uint32_t* nn;       // [n_e0]
+    float* w;           // [n_e0]
+    int32_t max_steps;  // stop after this many steps and dump
+    int32_t enabled;
+};
+
+struct LaunchArgs {
+    const RunDesc* runs;
+    const int32_t* c_run;   // [n_clusters] run index of each cluster
+    const int32_t* c_sim;   // [n_clusters] sim index within its run
+    int32_t n_clusters;
+    uint32_t key0, key1;    // Philox key = seed
+    int32_t n_bins;
+    uint32_t* counts;       // [n_clusters][n_bins]
+    uint64_t* live_steps;   // [n_clusters] or null
+    int32_t h_cap;          // LDS hole capacity (multiple of 4)
+    int32_t e_cap;          // LDS electron capacity (multiple of 64)
+    StateDump dump;
+};
+
+constexpr int kWaveHost = 64;   // wavefront width of gfx950
+
+// LDS bytes the fixed-dt kernel needs for the given capacities.
+inline int fixed_dt_lds_bytes(int h_cap, int e_cap)
+{
+    return 16 * h_cap + 20 * e_cap;
+}
+
+hipError_t launch_fixed_dt(const LaunchArgs& a, int lds_bytes, hipStream_t stream);
+hipError_t prepare_fixed_dt(int lds_bytes);
+
+}  // namespace mcl

@@ -1,0 +1,530 @@
+/*
+ * mcl_oracle.c — CPU replay harness for the fixed-dt trap-kinetics Monte Carlo.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke entry
+ * point and the CPU-baseline leg of bench.py may load liborc.so; the product
+ * (libmcl.so and the mcluminescence_amd package) never does.
+ *
+ * It restates, in plain C and independently of the HIP kernels, the algorithm
+ * of the reference's hot path with the fixed-dt step the kernels implement:
+ *   geometry       tl_trap_lab.py:33-39, engine.py:124-129
+ *   nearest hole   engine.py:113-119 (argmin, first index on ties), 154-175
+ *   rates          engine.py:65-77  (k = s e^{-E_cb/kT} + b e^{-E_loc/kT - alpha r},
+ *                                    E_loc = E_loc_2 if U < Retrap else E_loc_1)
+ *   step loop      simulate.py:470-516 (start-of-step temperature)
+ *   conflicts      src/est_params/functions.py:228-239 (first electron keeps the hole)
+ * Random numbers come from Philox4x32-10 (Salmon et al., SC'11) keyed by the
+ * run seed and counted by (step block, particle, run, sim|tag); transcendental
+ * functions are this file's own Cody-Waite + Taylor evaluations with explicit
+ * fma, so that every float32 decision is reproducible bit for bit.
+ *
+ * Mode fp64 = 1 keeps coordinates, distances and probabilities in float64 with
+ * libm exp/expm1: the all-FP64 reference against which the float32 curves'
+ * error is stated.  It consumes the same Philox stream.
+ *
+ * Parity pins (tests/): Philox against Random123 known answers and rocRAND's
+ * independent implementation; exp/expm1 against libm to <= 2 ulp; geometry
+ * against the closed forms; nearest-hole maintenance against brute force;
+ * ensembles against closed forms and the numpy event-mode restatement.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/mcl.h"
+
+/* ---------------------------------------------------------------- Philox -- */
+#define PH_M0 0xD2511F53u
+#define PH_M1 0xCD9E8D57u
+#define PH_W0 0x9E3779B9u
+#define PH_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) { k0 += PH_W0; k1 += PH_W1; }
+        uint64_t p0 = (uint64_t)PH_M0 * x0;
+        uint64_t p1 = (uint64_t)PH_M1 * x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+static inline uint32_t philox_word(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                   const uint32_t key[2], int word)
+{
+    uint32_t ctr[4] = {c0, c1, c2, c3}, out[4];
+    orc_philox4x32_10(ctr, key, out);
+    return out[word];
+}
+
+/* ------------------------------------------------------- own exp / expm1 -- */
+/* float32: x = k ln2 + r, |r| <= ln2/2, ln2 split in 12 + 24 bits (k*hi exact),
+ * expm1(r) = r + r^2 (1/2! + r/3! + ... + r^6/8!) by Horner with fma.          */
+#define F_INV_LN2 0x1.715476p+0f
+#define F_LN2_HI  0x1.62ep-1f
+#define F_LN2_LO  0x1.0bfbe8p-15f
+#define F_C2 0x1p-1f
+#define F_C3 0x1.555556p-3f
+#define F_C4 0x1.555556p-5f
+#define F_C5 0x1.111112p-7f
+#define F_C6 0x1.6c16c2p-10f
+#define F_C7 0x1.a01a02p-13f
+#define F_C8 0x1.a01a02p-16f
+
+static inline float f_pow2i(int k)   /* 2^k, -126 <= k <= 127 */
+{
+    uint32_t bits = (uint32_t)(k + 127) << 23;
+    float f;
+    memcpy(&f, &bits, sizeof f);
+    return f;
+}
+
+static inline float f_em1_core(float r)
+{
+    float p = F_C8;
+    p = fmaf(p, r, F_C7);
+    p = fmaf(p, r, F_C6);
+    p = fmaf(p, r, F_C5);
+    p = fmaf(p, r, F_C4);
+    p = fmaf(p, r, F_C3);
+    p = fmaf(p, r, F_C2);
+    float r2 = r * r;
+    return fmaf(r2, p, r);
+}
+
+float orc_expm1f(float x)
+{
+    if (x != x) return x;
+    if (x < -17.5f) return -1.0f;             /* e^x < 2^-25: rounds to -1 */
+    if (x > 88.0f) return INFINITY;
+    float kf = rintf(x * F_INV_LN2);
+    float r = fmaf(-kf, F_LN2_HI, x);
+    r = fmaf(-kf, F_LN2_LO, r);
+    float em1 = f_em1_core(r);
+    if (kf == 0.0f) return em1;
+    float t = f_pow2i((int)kf);
+    return fmaf(t, em1, t - 1.0f);
+}
+
+float orc_expf(float x)
+{
+    if (x != x) return x;
+    if (x < -87.0f) return 0.0f;              /* results below FLT_MIN flush to 0 */
+    if (x > 88.0f) return INFINITY;
+    float kf = rintf(x * F_INV_LN2);
+    float r = fmaf(-kf, F_LN2_HI, x);
+    r = fmaf(-kf, F_LN2_LO, r);
+    float e = 1.0f + f_em1_core(r);
+    return e * f_pow2i((int)kf);
+}
+
+/* float64: same reduction (ln2 split 21 + 53 bits), exp(r) Taylor to r^13/13!. */
+#define D_INV_LN2 0x1.71547652b82fep+0
+#define D_LN2_HI  0x1.62e42p-1
+#define D_LN2_LO  0x1.fdf473de6af28p-22
+
+static const double D_INVFACT[14] = {
+    0x1p+0, 0x1p+0, 0x1p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+    0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
+    0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22,
+    0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29, 0x1.6124613a86d09p-33,
+};
+
+static inline double d_pow2i(int k)  /* 2^k, -1022 <= k <= 1023 */
+{
+    uint64_t bits = (uint64_t)(k + 1023) << 52;
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    return d;
+}
+
+double orc_exp(double x)
+{
+    if (x != x) return x;
+    if (x < -708.0) return 0.0;               /* results below DBL_MIN flush to 0 */
+    if (x > 709.0) return INFINITY;
+    double kd = rint(x * D_INV_LN2);
+    double r = fma(-kd, D_LN2_HI, x);
+    r = fma(-kd, D_LN2_LO, r);
+    double p = D_INVFACT[13];
+    for (int i = 12; i >= 1; --i) p = fma(p, r, D_INVFACT[i]);
+    double e = fma(p, r, 1.0);
+    return e * d_pow2i((int)kd);
+}
+
+/* -------------------------------------------------------------- geometry -- */
+int orc_run_geometry(const mcl_run_params* p, mcl_geometry* g)
+{
+    /* tl_trap_lab.py:33-34: rho = rho' * (3/(4 pi) * alpha^3); d = (holes/rho)^(1/3) */
+    double rho = p->rho_prime * (3.0 / (4.0 * 0x1.921fb54442d18p+1) * pow(p->alpha, 3.0));
+    g->rho = rho;
+    g->side_m = pow(p->holes / rho, 1.0 / 3.0);
+    /* tl_trap_lab.py:38, engine.py:127: int() truncation */
+    g->n_e0 = (int32_t)(p->N_e * p->e_ratio_start);
+    g->n_h = (int32_t)(p->holes * pow(p->boundary_factor, 3.0));
+    return 0;
+}
+
+/* -------------------------------------------------------- cluster state --- */
+typedef struct {
+    int n_e0, n_h;
+    float *ex, *ey, *ez, *hx, *hy, *hz;        /* float32 mode */
+    double *dex, *dey, *dez, *dhx, *dhy, *dhz; /* float64 mode */
+    unsigned char* h_alive;
+    uint32_t* nn;
+    float* w;
+    double* dw;
+    int32_t* live;    /* live electron indices, ascending */
+    int n_live;
+} cluster_t;
+
+static void* xcalloc(size_t n, size_t sz) { void* p = calloc(n ? n : 1, sz); return p; }
+
+static void cluster_free(cluster_t* c)
+{
+    free(c->ex); free(c->ey); free(c->ez); free(c->hx); free(c->hy); free(c->hz);
+    free(c->dex); free(c->dey); free(c->dez); free(c->dhx); free(c->dhy); free(c->dhz);
+    free(c->h_alive); free(c->nn); free(c->w); free(c->dw); free(c->live);
+}
+
+/* nearest live hole of electron e: float32 squared distance
+ * d2 = fma(dz, dz, fma(dy, dy, dx*dx)); strict < keeps the first index on ties */
+static uint32_t nearest_f32(const cluster_t* c, int e, float* d2_out)
+{
+    float best = INFINITY;
+    uint32_t idx = MCL_NO_HOLE;
+    const float x = c->ex[e], y = c->ey[e], z = c->ez[e];
+    for (int h = 0; h < c->n_h; ++h) {
+        if (!c->h_alive[h]) continue;
+        float dx = c->hx[h] - x, dy = c->hy[h] - y, dz = c->hz[h] - z;
+        float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (d2 < best) { best = d2; idx = (uint32_t)h; }
+    }
+    *d2_out = best;
+    return idx;
+}
+
+static uint32_t nearest_f64(const cluster_t* c, int e, double* d2_out)
+{
+    double best = INFINITY;
+    uint32_t idx = MCL_NO_HOLE;
+    const double x = c->dex[e], y = c->dey[e], z = c->dez[e];
+    for (int h = 0; h < c->n_h; ++h) {
+        if (!c->h_alive[h]) continue;
+        double dx = c->dhx[h] - x, dy = c->dhy[h] - y, dz = c->dhz[h] - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < best) { best = d2; idx = (uint32_t)h; }
+    }
+    *d2_out = best;
+    return idx;
+}
+
+static void refresh_nn(cluster_t* c, int e, float alpha_f, double alpha_nm, int fp64)
+{
+    if (fp64) {
+        double d2;
+        c->nn[e] = nearest_f64(c, e, &d2);
+        c->dw[e] = (c->nn[e] == MCL_NO_HOLE) ? 0.0 : exp(-(alpha_nm * sqrt(d2)));
+    } else {
+        float d2;
+        c->nn[e] = nearest_f32(c, e, &d2);
+        c->w[e] = (c->nn[e] == MCL_NO_HOLE) ? 0.0f : orc_expf(-(alpha_f * sqrtf(d2)));
+    }
+}
+
+static int cluster_init(cluster_t* c, const mcl_run_params* p, int run, int sim,
+                        const uint32_t key[2], int fp64)
+{
+    mcl_geometry g;
+    orc_run_geometry(p, &g);
+    memset(c, 0, sizeof *c);
+    c->n_e0 = g.n_e0 > 0 ? g.n_e0 : 0;
+    c->n_h = g.n_h > 0 ? g.n_h : 0;
+    const double side_nm = g.side_m * 1e9;
+    const double side_b_nm = (g.side_m * p->boundary_factor) * 1e9;
+    const uint32_t c3e = ((uint32_t)sim << 4) | MCL_TAG_GEOM_E;
+    const uint32_t c3h = ((uint32_t)sim << 4) | MCL_TAG_GEOM_H;
+    int ne = c->n_e0, nh = c->n_h;
+    c->h_alive = xcalloc(nh, 1);
+    c->nn = xcalloc(ne, sizeof(uint32_t));
+    c->live = xcalloc(ne, sizeof(int32_t));
+    if (fp64) {
+        c->dex = xcalloc(ne, 8); c->dey = xcalloc(ne, 8); c->dez = xcalloc(ne, 8);
+        c->dhx = xcalloc(nh, 8); c->dhy = xcalloc(nh, 8); c->dhz = xcalloc(nh, 8);
+        c->dw = xcalloc(ne, 8);
+    } else {
+        c->ex = xcalloc(ne, 4); c->ey = xcalloc(ne, 4); c->ez = xcalloc(ne, 4);
+        c->hx = xcalloc(nh, 4); c->hy = xcalloc(nh, 4); c->hz = xcalloc(nh, 4);
+        c->w = xcalloc(ne, 4);
+    }
+    for (int e = 0; e < ne; ++e) {
+        uint32_t ctr[4] = {0u, (uint32_t)e, (uint32_t)run, c3e}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        double ux = (double)o[0] * 0x1p-32, uy = (double)o[1] * 0x1p-32, uz = (double)o[2] * 0x1p-32;
+        if (fp64) { c->dex[e] = ux * side_nm; c->dey[e] = uy * side_nm; c->dez[e] = uz * side_nm; }
+        else { c->ex[e] = (float)(ux * side_nm); c->ey[e] = (float)(uy * side_nm); c->ez[e] = (float)(uz * side_nm); }
+        c->live[e] = e;
+    }
+    for (int h = 0; h < nh; ++h) {
+        uint32_t ctr[4] = {0u, (uint32_t)h, (uint32_t)run, c3h}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        double ux = (double)o[0] * 0x1p-32, uy = (double)o[1] * 0x1p-32, uz = (double)o[2] * 0x1p-32;
+        if (fp64) { c->dhx[h] = ux * side_b_nm; c->dhy[h] = uy * side_b_nm; c->dhz[h] = uz * side_b_nm; }
+        else { c->hx[h] = (float)(ux * side_b_nm); c->hy[h] = (float)(uy * side_b_nm); c->hz[h] = (float)(uz * side_b_nm); }
+        c->h_alive[h] = 1;
+    }
+    c->n_live = ne;
+    const double alpha_nm = p->alpha * 1e-9;
+    const float alpha_f = (float)alpha_nm;
+    for (int e = 0; e < ne; ++e) refresh_nn(c, e, alpha_f, alpha_nm, fp64);
+    return 0;
+}
+
+/* ------------------------------------------------------------ step rule --- */
+/* Fire threshold for k*dt = x in float32: p = -expm1(-x); fire iff p >= 1 or
+ * u32 < (uint32)(p * 2^32). */
+static inline void thr_f32(float x, uint32_t* thr, int* always)
+{
+    float p = -orc_expm1f(-x);
+    *always = p >= 1.0f;
+    *thr = *always ? 0xFFFFFFFFu : (uint32_t)(p * 0x1p32f);
+}
+
+static inline void thr_f64(double x, uint64_t* thr, int* always)
+{
+    double p = -expm1(-x);
+    *always = p >= 1.0;
+    *thr = *always ? 0xFFFFFFFFull : (uint64_t)(p * 0x1p32);
+}
+
+static int validate(const mcl_run_params* p, int n_bins)
+{
+    if (!(p->dt > 0) || p->n_steps <= 0 || p->sims < 0) return MCL_EINVAL;
+    if (n_bins <= 0 || n_bins > p->n_steps) return MCL_EOVERFLOW;
+    if (p->D != 0.0) return MCL_EINVAL;
+    if (!(p->k_b > 0) || !(p->T_start + 273.15 > 0)) return MCL_EINVAL;
+    mcl_geometry g;
+    orc_run_geometry(p, &g);
+    if (g.n_e0 > 65535 || g.n_h > 65534) return MCL_ERANGE;
+    return 0;
+}
+
+/* Final state of a cluster after a bounded number of steps (test hook). */
+typedef struct {
+    int max_steps;            /* stop after this many steps                     */
+    unsigned char* e_live;    /* [n_e0] 1 if the electron is still trapped      */
+    unsigned char* h_alive;   /* [n_h]                                          */
+    uint32_t* nn;             /* [n_e0] maintained nearest hole                 */
+    float* e_xyz;             /* [n_e0][3]                                      */
+    float* h_xyz;             /* [n_h][3]                                       */
+} final_state_t;
+
+/* Simulate one cluster; counts[n_bins] (zeroed here), returns live electron-steps. */
+static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uint32_t key[2],
+                            int n_bins, int fp64, uint32_t* counts, final_state_t* fs)
+{
+    cluster_t c;
+    cluster_init(&c, p, run, sim, key, fp64);
+    memset(counts, 0, sizeof(uint32_t) * (size_t)n_bins);
+    const int n_steps = p->n_steps;
+    const double T0_K = p->T_start + 273.15;
+    const double alpha_nm = p->alpha * 1e-9;
+    const float alpha_f = (float)alpha_nm;
+    const int two_channel = (p->E_loc_1 != p->E_loc_2) && (p->Retrap > 0.0) && (p->Retrap < 1.0);
+    const int always_chan2 = p->Retrap >= 1.0;
+    const uint64_t ret_thr = two_channel ? (uint64_t)(p->Retrap * 0x1p32) : 0;
+    const uint32_t c3f = ((uint32_t)sim << 4) | MCL_TAG_FIRE;
+    const uint32_t c3c = ((uint32_t)sim << 4) | MCL_TAG_CHAN;
+
+    int32_t* fired = xcalloc(c.n_e0, sizeof(int32_t));
+    uint32_t* claim = xcalloc(c.n_h, sizeof(uint32_t));
+    for (int h = 0; h < c.n_h; ++h) claim[h] = 0xFFFFFFFFu;
+    unsigned char* won = xcalloc(c.n_e0, 1);
+    uint64_t live_steps = 0;
+
+    const int step_limit = (fs && fs->max_steps < n_steps) ? fs->max_steps : n_steps;
+    for (int n = 0; n < step_limit && c.n_live > 0; ++n) {
+        live_steps += (uint64_t)c.n_live;
+        /* per-step float64 scalars (simulate.py:477-479, engine.py:65-73) */
+        const double t = (double)n * p->dt;
+        const double T = T0_K + p->T_rate * t;
+        const double kT = p->k_b * T;
+        double A, B1, B2;
+        if (fp64) {
+            A = p->s * exp(-(p->E_cb / kT));
+            B1 = p->b * exp(-(p->E_loc_1 / kT));
+            B2 = p->b * exp(-(p->E_loc_2 / kT));
+        } else {
+            A = p->s * orc_exp(-(p->E_cb / kT));
+            B1 = p->b * orc_exp(-(p->E_loc_1 / kT));
+            B2 = p->b * orc_exp(-(p->E_loc_2 / kT));
+        }
+        const uint32_t blk = (uint32_t)n >> 2;
+        const int word = n & 3;
+        const float xA = (float)(A * p->dt);
+        const float b1dt = (float)(B1 * p->dt), b2dt = (float)(B2 * p->dt);
+        int n_fired = 0;
+        for (int i = 0; i < c.n_live; ++i) {
+            const int e = c.live[i];
+            if (c.nn[e] == MCL_NO_HOLE) continue;
+            int chan2 = always_chan2;
+            if (two_channel) {
+                uint32_t uc = philox_word(blk, (uint32_t)e, (uint32_t)run, c3c, key, word);
+                chan2 = (uint64_t)uc < ret_thr;
+            }
+            const uint32_t u = philox_word(blk, (uint32_t)e, (uint32_t)run, c3f, key, word);
+            int fire;
+            if (fp64) {
+                const double x = A * p->dt + (chan2 ? B2 : B1) * p->dt * c.dw[e];
+                uint64_t thr; int always;
+                thr_f64(x, &thr, &always);
+                fire = always || (uint64_t)u < thr;
+            } else {
+                const float x = fmaf(chan2 ? b2dt : b1dt, c.w[e], xA);
+                uint32_t thr; int always;
+                thr_f32(x, &thr, &always);
+                fire = always || u < thr;
+            }
+            if (fire) fired[n_fired++] = e;   /* ascending electron index */
+        }
+        if (n_fired == 0) continue;
+        /* conflict rule: the lowest electron index keeps a shared hole */
+        int wins = 0;
+        for (int j = 0; j < n_fired; ++j) {
+            const int e = fired[j];
+            const uint32_t h = c.nn[e];
+            if (claim[h] == 0xFFFFFFFFu) { claim[h] = (uint32_t)e; won[e] = 1; ++wins; }
+        }
+        for (int j = 0; j < n_fired; ++j) {
+            const int e = fired[j];
+            if (won[e]) { c.h_alive[c.nn[e]] = 0; }
+            claim[c.nn[e]] = 0xFFFFFFFFu;
+        }
+        /* remove winners (stable), re-search electrons whose hole died */
+        int k = 0;
+        for (int i = 0; i < c.n_live; ++i) {
+            const int e = c.live[i];
+            if (won[e]) { won[e] = 0; continue; }
+            c.live[k++] = e;
+        }
+        c.n_live = k;
+        for (int i = 0; i < c.n_live; ++i) {
+            const int e = c.live[i];
+            if (c.nn[e] != MCL_NO_HOLE && !c.h_alive[c.nn[e]])
+                refresh_nn(&c, e, alpha_f, alpha_nm, fp64);
+        }
+        const int bin = (int)(((int64_t)n * n_bins) / n_steps);
+        counts[bin] += (uint32_t)wins;
+    }
+    if (fs) {
+        for (int e = 0; e < c.n_e0; ++e) fs->e_live[e] = 0;
+        for (int i = 0; i < c.n_live; ++i) fs->e_live[c.live[i]] = 1;
+        for (int h = 0; h < c.n_h; ++h) fs->h_alive[h] = c.h_alive[h];
+        for (int e = 0; e < c.n_e0; ++e) fs->nn[e] = c.nn[e];
+        if (!fp64) {
+            for (int e = 0; e < c.n_e0; ++e) {
+                fs->e_xyz[3 * e] = c.ex[e]; fs->e_xyz[3 * e + 1] = c.ey[e]; fs->e_xyz[3 * e + 2] = c.ez[e];
+            }
+            for (int h = 0; h < c.n_h; ++h) {
+                fs->h_xyz[3 * h] = c.hx[h]; fs->h_xyz[3 * h + 1] = c.hy[h]; fs->h_xyz[3 * h + 2] = c.hz[h];
+            }
+        }
+    }
+    free(fired); free(claim); free(won);
+    cluster_free(&c);
+    return live_steps;
+}
+
+/* counts [total_clusters][n_bins]; live_steps [total_clusters] or NULL. */
+int orc_run_fixed_dt(const mcl_run_params* runs, int32_t n_runs, uint64_t seed,
+                     int32_t n_bins, int32_t fp64, int32_t n_threads,
+                     uint32_t* counts, uint64_t* live_steps)
+{
+    if (!runs || n_runs <= 0 || !counts) return MCL_EINVAL;
+    int64_t total = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        int st = validate(&runs[r], n_bins);
+        if (st) return st;
+        total += runs[r].sims;
+    }
+    int32_t* c_run = xcalloc((size_t)total, sizeof(int32_t));
+    int32_t* c_sim = xcalloc((size_t)total, sizeof(int32_t));
+    int64_t k = 0;
+    for (int r = 0; r < n_runs; ++r)
+        for (int s = 0; s < runs[r].sims; ++s) { c_run[k] = r; c_sim[k] = s; ++k; }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t i = 0; i < total; ++i) {
+        uint64_t ls = run_cluster(&runs[c_run[i]], c_run[i], c_sim[i], key, n_bins, fp64,
+                                  counts + (size_t)i * (size_t)n_bins, NULL);
+        if (live_steps) live_steps[i] = ls;
+    }
+    free(c_run); free(c_sim);
+    return 0;
+}
+
+/* Initial float32 state of one cluster (coordinates in nm, nn, w). */
+int orc_init_state(const mcl_run_params* p, int32_t run, int32_t sim, uint64_t seed,
+                   float* e_xyz, float* h_xyz, uint32_t* nn, float* w)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    cluster_t c;
+    cluster_init(&c, p, run, sim, key, 0);
+    for (int e = 0; e < c.n_e0; ++e) {
+        e_xyz[3 * e] = c.ex[e]; e_xyz[3 * e + 1] = c.ey[e]; e_xyz[3 * e + 2] = c.ez[e];
+        nn[e] = c.nn[e]; w[e] = c.w[e];
+    }
+    for (int h = 0; h < c.n_h; ++h) {
+        h_xyz[3 * h] = c.hx[h]; h_xyz[3 * h + 1] = c.hy[h]; h_xyz[3 * h + 2] = c.hz[h];
+    }
+    cluster_free(&c);
+    return 0;
+}
+
+/* State of cluster (run, sim) after max_steps float32 steps: which electrons
+ * and holes are left and the incrementally maintained nearest hole. */
+int orc_final_state(const mcl_run_params* p, int32_t run, int32_t sim, uint64_t seed,
+                    int32_t max_steps, int32_t n_bins, uint32_t* counts,
+                    unsigned char* e_live, unsigned char* h_alive, uint32_t* nn,
+                    float* e_xyz, float* h_xyz)
+{
+    int st = validate(p, n_bins);
+    if (st) return st;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    final_state_t fs = {max_steps, e_live, h_alive, nn, e_xyz, h_xyz};
+    run_cluster(p, run, sim, key, n_bins, 0, counts, &fs);
+    return 0;
+}
+
+/* Brute-force checker for nearest-hole maintenance: given coordinates and an
+ * alive mask, recompute every electron's nearest hole from scratch. */
+void orc_bruteforce_nn(const float* e_xyz, int n_e, const float* h_xyz, const unsigned char* alive,
+                       int n_h, uint32_t* nn)
+{
+    for (int e = 0; e < n_e; ++e) {
+        float best = INFINITY;
+        uint32_t idx = MCL_NO_HOLE;
+        for (int h = 0; h < n_h; ++h) {
+            if (!alive[h]) continue;
+            float dx = h_xyz[3 * h] - e_xyz[3 * e], dy = h_xyz[3 * h + 1] - e_xyz[3 * e + 1];
+            float dz = h_xyz[3 * h + 2] - e_xyz[3 * e + 2];
+            float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            if (d2 < best) { best = d2; idx = (uint32_t)h; }
+        }
+        nn[e] = idx;
+    }
+}

@@ -1,0 +1,136 @@
+"""ctypes binding of liborc.so (oracle/mcl_oracle.c): the CPU replay harness.
+
+TEST INFRASTRUCTURE ONLY — the checker for the HIP kernels, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+from mcluminescence_amd._abi import DEFAULT_SEED, Geometry, RunParams, run_array
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_philox4x32_10.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_philox4x32_10.restype = None
+        L.orc_expm1f.argtypes = [C.c_float]
+        L.orc_expm1f.restype = C.c_float
+        L.orc_expf.argtypes = [C.c_float]
+        L.orc_expf.restype = C.c_float
+        L.orc_exp.argtypes = [C.c_double]
+        L.orc_exp.restype = C.c_double
+        L.orc_run_geometry.argtypes = [C.POINTER(RunParams), C.POINTER(Geometry)]
+        L.orc_run_fixed_dt.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_uint64, C.c_int32,
+                                       C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+        L.orc_init_state.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_final_state.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
+                                      C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_bruteforce_nn.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_void_p]
+        L.orc_bruteforce_nn.restype = None
+        _lib = L
+    return _lib
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def expm1f(x: float) -> np.float32:
+    return np.float32(lib().orc_expm1f(float(x)))
+
+
+def expf(x: float) -> np.float32:
+    return np.float32(lib().orc_expf(float(x)))
+
+
+def exp64(x: float) -> float:
+    return float(lib().orc_exp(float(x)))
+
+
+def geometry(run: RunParams) -> Geometry:
+    g = Geometry()
+    lib().orc_run_geometry(C.byref(run), C.byref(g))
+    return g
+
+
+def run_fixed_dt(runs: Sequence[RunParams], seed: int = DEFAULT_SEED, n_bins=None,
+                 fp64: bool = False, threads: int = 0):
+    """Replay harness: counts [clusters, n_bins] uint32 and live_steps [clusters] uint64."""
+    arr = run_array(runs)
+    if n_bins is None:
+        n_bins = min(int(r.n_steps) for r in runs)
+    total = sum(int(r.sims) for r in runs)
+    counts = np.zeros((total, n_bins), np.uint32)
+    live = np.zeros(total, np.uint64)
+    st = lib().orc_run_fixed_dt(arr, len(runs), C.c_uint64(seed), int(n_bins), int(bool(fp64)),
+                                int(threads), counts.ctypes.data, live.ctypes.data)
+    if st != 0:
+        raise ValueError(f"oracle rejected the runs (status {st})")
+    return counts, live
+
+
+def init_state(run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED) -> dict:
+    g = geometry(run)
+    ne, nh = g.n_e0, g.n_h
+    e_xyz = np.zeros((ne, 3), np.float32)
+    h_xyz = np.zeros((nh, 3), np.float32)
+    nn = np.zeros(ne, np.uint32)
+    w = np.zeros(ne, np.float32)
+    lib().orc_init_state(C.byref(run), run_index, sim, C.c_uint64(seed), e_xyz.ctypes.data,
+                         h_xyz.ctypes.data, nn.ctypes.data, w.ctypes.data)
+    return dict(e_xyz=e_xyz, h_xyz=h_xyz, nn=nn, w=w)
+
+
+def final_state(run: RunParams, run_index: int, sim: int, max_steps: int,
+                seed: int = DEFAULT_SEED) -> dict:
+    g = geometry(run)
+    ne, nh = g.n_e0, g.n_h
+    counts = np.zeros(int(run.n_steps), np.uint32)
+    e_live = np.zeros(ne, np.uint8)
+    h_alive = np.zeros(nh, np.uint8)
+    nn = np.zeros(ne, np.uint32)
+    e_xyz = np.zeros((ne, 3), np.float32)
+    h_xyz = np.zeros((nh, 3), np.float32)
+    st = lib().orc_final_state(C.byref(run), run_index, sim, C.c_uint64(seed), int(max_steps),
+                               int(run.n_steps), counts.ctypes.data, e_live.ctypes.data,
+                               h_alive.ctypes.data, nn.ctypes.data, e_xyz.ctypes.data,
+                               h_xyz.ctypes.data)
+    if st != 0:
+        raise ValueError(f"oracle rejected the run (status {st})")
+    return dict(counts=counts, e_live=e_live, h_alive=h_alive, nn=nn, e_xyz=e_xyz, h_xyz=h_xyz)
+
+
+def bruteforce_nn(e_xyz: np.ndarray, h_xyz: np.ndarray, h_alive: np.ndarray) -> np.ndarray:
+    e = np.ascontiguousarray(e_xyz, np.float32)
+    h = np.ascontiguousarray(h_xyz, np.float32)
+    a = np.ascontiguousarray(h_alive, np.uint8)
+    nn = np.zeros(len(e), np.uint32)
+    lib().orc_bruteforce_nn(e.ctypes.data, len(e), h.ctypes.data, a.ctypes.data, len(h),
+                            nn.ctypes.data)
+    return nn

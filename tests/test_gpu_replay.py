@@ -1,0 +1,87 @@
+"""Bit-exact replay: the HIP kernel against the CPU replay harness (oracle/mcl_oracle.c).
+
+Both consume the same Philox4x32-10 stream per (run, sim, electron, step) and the
+same float32/float64 arithmetic, so per-step recombination counts must agree
+exactly, cluster by cluster.
+"""
+import numpy as np
+import pytest
+
+from mcluminescence_amd import workloads as W
+from mcluminescence_amd._abi import RunParams
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _replay(ctx, runs, n_bins=None, seed=0x4D434C554D000001):
+    print(f"replay: {len(runs)} runs, {sum(r.sims for r in runs)} clusters, seed {seed:#x}",
+          flush=True)
+    g_counts, g_live, stats = ctx.run_fixed_dt(runs, seed=seed, n_bins=n_bins)
+    o_counts, o_live = orc.run_fixed_dt(runs, seed=seed, n_bins=n_bins, threads=8)
+    return g_counts, g_live, o_counts, o_live, stats
+
+
+def _assert_same(g_counts, g_live, o_counts, o_live):
+    bad = np.nonzero((g_counts != o_counts).any(axis=1))[0]
+    assert bad.size == 0, (
+        f"{bad.size} clusters differ; first {bad[0]}: first step "
+        f"{np.nonzero(g_counts[bad[0]] != o_counts[bad[0]])[0][:5]}")
+    np.testing.assert_array_equal(g_live, o_live)
+
+
+def test_initial_state_matches_oracle(gpu_ctx):
+    run = W.tl_glow_16x500(1)[0]
+    for run_index, sim in [(0, 0), (3, 7)]:
+        g = gpu_ctx.cluster_state(run, run_index, sim)
+        o = orc.init_state(run, run_index, sim)
+        np.testing.assert_array_equal(g["e_xyz"], o["e_xyz"])
+        np.testing.assert_array_equal(g["h_xyz"], o["h_xyz"])
+        np.testing.assert_array_equal(g["nn"], o["nn"])
+        np.testing.assert_array_equal(g["w"].view(np.uint32), o["w"].view(np.uint32))
+        assert g["e_live"].all() and g["h_alive"].all()
+
+
+def test_tl_glow_per_step_bit_exact(gpu_ctx):
+    runs = W.tl_glow_16x500()
+    g, gl, o, ol, stats = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+    assert g.sum() == 16 * 500
+    assert stats.live_electron_steps == int(ol.sum())
+
+
+def test_iso_decay_per_step_bit_exact(gpu_ctx):
+    runs = W.iso_decay_4096x400(sims=256)
+    g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+
+
+def test_tunnelling_per_step_bit_exact(gpu_ctx):
+    # reference TL12/basicTL12 physics: every electron-step takes the per-electron
+    # threshold path (the band term is zero)
+    runs = W.tunnelling_tl12(sims=4, n=600, T_rate=5.0)
+    g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+    assert g.sum() > 0
+
+
+def test_two_channel_and_binned_bit_exact(gpu_ctx):
+    r = W.tunnelling_tl12(sims=3, n=300, T_rate=5.0)[0]
+    r.E_loc_1, r.E_loc_2, r.Retrap = 1.15, 1.25, 0.3
+    r2 = W.tl_glow_16x500(sims=5)[0]
+    r2.n_steps = r.n_steps
+    g, gl, o, ol, _ = _replay(gpu_ctx, [r, r2], n_bins=40)
+    _assert_same(g, gl, o, ol)
+
+
+def test_state_after_steps_matches_oracle_and_bruteforce(gpu_ctx):
+    run = W.tunnelling_tl12(sims=1, n=500, T_rate=5.0)[0]
+    for steps in (40, 100):
+        g = gpu_ctx.cluster_state(run, 0, 0, max_steps=steps)
+        o = orc.final_state(run, 0, 0, steps)
+        np.testing.assert_array_equal(g["e_live"], o["e_live"])
+        np.testing.assert_array_equal(g["h_alive"], o["h_alive"])
+        live = g["e_live"].astype(bool)
+        np.testing.assert_array_equal(g["nn"][live], o["nn"][live])
+        bf = orc.bruteforce_nn(o["e_xyz"][live], o["h_xyz"], o["h_alive"])
+        np.testing.assert_array_equal(g["nn"][live], bf)
