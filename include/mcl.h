@@ -49,6 +49,12 @@ extern "C" {
 #define MCL_TAG_GEOM_H 1u   /* hole coordinates, counter (0, h, run, sim|tag)       */
 #define MCL_TAG_FIRE   2u   /* recombination draw, counter (n>>2, e, run, sim|tag)  */
 #define MCL_TAG_CHAN   3u   /* localisation-channel draw, same counter layout       */
+#define MCL_TAG_RETRY0 4u   /* conflict retry after round r: tag RETRY0 + r         */
+
+/* Conflict-resolution rounds per step: round 0 resolves the electrons that
+ * fired; each later round retries the previous round's losers on their new
+ * nearest hole.  Losers left after the last round wait for the next step. */
+#define MCL_MAX_ROUNDS 12
 
 /* Index sentinel for "no live hole" (n_h reached 0). */
 #define MCL_NO_HOLE 0xFFFFu
@@ -146,6 +152,18 @@ int  mcl_plan_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs,
                        uint64_t seed, int32_t n_bins, uint32_t flags, void** plan_out);
 int  mcl_plan_info_get(const void* plan, mcl_plan_info* info);
 int  mcl_plan_launch(void* plan, uint32_t* counts_dev, uint64_t* live_steps_dev);
+
+/* Work done by the last launch of a plan, summed over its clusters (waits for
+ * the launch): the algorithmic units behind throughput and roofline figures. */
+typedef struct mcl_work {
+    uint64_t live_electron_steps;  /* live electrons summed over every simulated step   */
+    uint64_t distance_evals;       /* electron-hole squared distances evaluated          */
+    uint64_t recombinations;       /* electron-hole pairs removed                        */
+    uint64_t cluster_steps;        /* steps simulated until each cluster emptied         */
+    double   kernel_ms;            /* device time of the launch (HIP events)             */
+} mcl_work;
+
+int  mcl_plan_work(void* plan, mcl_work* out);
 /* Device time of the last launch of this plan (HIP events on the launch stream). */
 int  mcl_plan_last_ms(void* plan, double* ms);
 void mcl_plan_destroy(void* plan);

@@ -85,6 +85,12 @@ class PlanInfo(C.Structure):
                 ("block", C.c_int32)]
 
 
+class Work(C.Structure):
+    _fields_ = [("live_electron_steps", C.c_uint64), ("distance_evals", C.c_uint64),
+                ("recombinations", C.c_uint64), ("cluster_steps", C.c_uint64),
+                ("kernel_ms", C.c_double)]
+
+
 class MclError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mcl error {code}: {msg}")
@@ -107,6 +113,7 @@ _SIGNATURES = {
     "mcl_plan_info_get": (C.c_int, [_P, C.POINTER(PlanInfo)]),
     "mcl_plan_launch": (C.c_int, [_P, _P, _P]),
     "mcl_plan_last_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mcl_plan_work": (C.c_int, [_P, C.POINTER(Work)]),
     "mcl_plan_destroy": (None, [_P]),
     "mcl_sync": (C.c_int, [_P]),
     "mcl_cluster_state": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
@@ -256,6 +263,11 @@ class Plan:
         ms = C.c_double()
         self.ctx.check(self.ctx.lib.mcl_plan_last_ms(self.handle, C.byref(ms)), "mcl_plan_last_ms")
         return ms.value
+
+    def work(self) -> Work:
+        w = Work()
+        self.ctx.check(self.ctx.lib.mcl_plan_work(self.handle, C.byref(w)), "mcl_plan_work")
+        return w
 
     def sync(self) -> None:
         self.ctx.check(self.ctx.lib.mcl_sync(self.ctx.handle), "mcl_sync")

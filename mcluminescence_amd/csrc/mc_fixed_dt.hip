@@ -9,6 +9,7 @@
 //   ex, ey, ez [e_cap]   electron coordinates, indexed by original electron index
 //   est        [e_cap]   compacted live electrons: (nearest hole << 16) | electron index
 //   ew         [e_cap]   compacted w = exp(-alpha r_nn), float32
+//   ethr       [e_cap]   fire threshold of a candidate in the current step (conflict retries)
 //
 // Per step n (start-of-step temperature T_n = T0 + T_rate n dt, simulate.py:477-479):
 //   scalars   A = s e^{-E_cb/kT}, B_i = b e^{-E_loc_i/kT} in float64 (own exp),
@@ -23,6 +24,10 @@
 //   remove    winners' slots become dead, their holes get the x sentinel.
 //   repair    survivors whose hole died re-search: one wave scan over the holes,
 //             (d2 bits, index) min-reduced across the wave = first-index argmin.
+//   retry     a conflict loser retries on its new nearest hole within the step,
+//             accepted with probability T_new / T_old (one more Philox word), for
+//             up to MCL_MAX_ROUNDS rounds; this removes the O(k dt) delay a
+//             "loser waits a step" rule would put into the decay curves.
 //   compact   stable ballot/mbcnt compaction, only when it drops a 64-slot row.
 // Counts go out per bin (bin of step n = n * n_bins / n_steps), zero-padded
 // after the last electron recombines, like the reference's padded rows.
@@ -60,6 +65,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     float* ez = ey + e_cap;
     uint32_t* est = (uint32_t*)(ez + e_cap);
     float* ew = (float*)(est + e_cap);
+    uint32_t* ethr = (uint32_t*)(ew + e_cap);
 
     const uint32_t k0 = a.key0, k1 = a.key1;
     const uint32_t prun = (uint32_t)R.run_id;
@@ -137,6 +143,8 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
 
     int n_live = n_e0, n_slots = n_e0;
     uint64_t lsteps = 0;
+    uint64_t dist_evals = (uint64_t)n_e0 * (uint64_t)n_h;
+    int steps_run = 0;
     uint32_t bin_acc = 0;
     int bin = 0;
     int64_t bin_rem = 0;
@@ -146,6 +154,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     for (int n = 0; n < step_limit; ++n) {
         if (n_live == 0) break;
         lsteps += (uint64_t)n_live;
+        steps_run = n + 1;
         if (n == 0 || R.T_rate != 0.0) {
             const double t = (double)n * R.dt;
             const double T = R.T0_K + R.T_rate * t;
@@ -162,8 +171,8 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         const int word = n & 3;
         const int rows = (n_slots + kWave - 1) / kWave;
 
-        // fire test + claims
-        uint64_t fbits = 0;
+        // round 0: fire test + claims
+        uint64_t cand = 0;
         for (int r = 0; r < rows; ++r) {
             const int p = r * kWave + lane;
             const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
@@ -182,70 +191,131 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                 if (x != xA) t = fire_threshold(x);
                 if (t.always || u < t.thr) {
                     atomicMin(&claim[nn], o);
-                    fbits |= 1ull << r;
+                    ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
+                    cand |= 1ull << r;
                 }
             }
         }
-        const uint64_t rowmask = wave_or_u64(fbits);
+        uint64_t rowmask = wave_or_u64(cand);
         uint32_t wins = 0;
         if (rowmask != 0) {
-            __syncthreads();
-            // winners: claim[nn] holds the lowest firing index for that hole
-            uint64_t m = rowmask;
-            while (m != 0) {
-                const int r = __builtin_ctzll(m);
-                m &= m - 1;
-                const int p = r * kWave + lane;
-                bool win = false;
-                uint32_t nn = 0;
-                if ((fbits >> r) & 1ull) {
-                    const uint32_t st = est[p];
-                    nn = st >> 16;
-                    win = claim[nn] == (st & 0xFFFFu);
+            for (int round = 0;; ++round) {
+                if (round > 0) {
+                    // claims of this round's candidates (losers accepted for a retry)
+                    uint64_t m = rowmask;
+                    while (m != 0) {
+                        const int r = __builtin_ctzll(m);
+                        m &= m - 1;
+                        if ((cand >> r) & 1ull) {
+                            const uint32_t st = est[r * kWave + lane];
+                            atomicMin(&claim[st >> 16], st & 0xFFFFu);
+                        }
+                    }
                 }
-                if (win) {
-                    claim[nn] = 0xFFFFFFFFu;
-                    hx[nn] = kDeadHoleX;
-                    est[p] = kDeadSlot;
+                __syncthreads();
+                // winners: claim[nn] holds the lowest candidate index for that hole
+                uint64_t lose = 0;
+                uint64_t m = rowmask;
+                while (m != 0) {
+                    const int r = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int p = r * kWave + lane;
+                    const bool is_cand = (cand >> r) & 1ull;
+                    bool win = false;
+                    uint32_t nn = 0;
+                    if (is_cand) {
+                        const uint32_t st = est[p];
+                        nn = st >> 16;
+                        win = claim[nn] == (st & 0xFFFFu);
+                        if (!win) lose |= 1ull << r;
+                    }
+                    if (win) {
+                        claim[nn] = 0xFFFFFFFFu;
+                        hx[nn] = kDeadHoleX;
+                        est[p] = kDeadSlot;
+                    }
+                    wins += (uint32_t)__popcll(ballot(win));
                 }
-                wins += (uint32_t)__popcll(ballot(win));
+                __syncthreads();
+
+                // repair: survivors whose nearest hole was removed re-search
+                for (int r = 0; r < rows; ++r) {
+                    const int p = r * kWave + lane;
+                    const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
+                    const uint32_t nn = st >> 16;
+                    const uint32_t nn_safe = nn != kNoHole ? nn : 0u;
+                    const bool need = nn != kNoHole && hx[nn_safe] == kDeadHoleX;
+                    uint64_t mm = ballot(need);
+                    while (mm != 0) {
+                        const int l = __builtin_ctzll(mm);
+                        mm &= mm - 1;
+                        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)st, l) & 0xFFFFu;
+                        const float x = ex[o], y = ey[o], z = ez[o];
+                        uint64_t key = ~0ull;
+                        for (int h = lane; h < n_h4; h += kWave) {
+                            const float dx = hx[h] - x, dy = hy[h] - y, dz = hz[h] - z;
+                            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                            const uint64_t k = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)h;
+                            key = k < key ? k : key;
+                        }
+                        key = wave_min_u64(key);
+                        dist_evals += (uint64_t)n_h;
+                        const uint32_t d2bits = (uint32_t)(key >> 32);
+                        const bool none = d2bits >= 0x7f800000u;
+                        const uint32_t nn_new = none ? kNoHole : (uint32_t)key;
+                        const float w =
+                            none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
+                        if (lane == 0) {
+                            est[r * kWave + l] = (nn_new << 16) | o;
+                            ew[r * kWave + l] = w;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (round + 1 == MCL_MAX_ROUNDS) break;
+
+                // a loser retries on its new nearest hole in the same step, accepted
+                // with probability T_new / T_old (thinning of the draw it already won)
+                const uint64_t lmask = wave_or_u64(lose);
+                if (lmask == 0) break;
+                cand = 0;
+                const uint32_t c3r = psim | (MCL_TAG_RETRY0 + (uint32_t)round);
+                m = lmask;
+                while (m != 0) {
+                    const int r = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int p = r * kWave + lane;
+                    if ((lose >> r) & 1ull) {
+                        const uint32_t st = est[p];
+                        const uint32_t o = st & 0xFFFFu, nn = st >> 16;
+                        if (nn != kNoHole) {
+                            bool chan2 = always2;
+                            if (two_ch) {
+                                const u32x4 rc = philox10(blk, o, prun, c3c, k0, k1);
+                                chan2 = (uint64_t)pick_word(rc, word) < ret_thr;
+                            }
+                            const float x = __builtin_fmaf(chan2 ? b2dt : b1dt, ew[p], xA);
+                            FireThr t = tA;
+                            if (x != xA) t = fire_threshold(x);
+                            const uint64_t Tn = t.always ? 0x100000000ull : (uint64_t)t.thr;
+                            const uint32_t eo = ethr[p];
+                            const uint64_t To = eo == 0xFFFFFFFFu ? 0x100000000ull : (uint64_t)eo;
+                            bool acc = Tn >= To;
+                            if (!acc) {
+                                const u32x4 rr = philox10(blk, o, prun, c3r, k0, k1);
+                                acc = (uint64_t)pick_word(rr, word) * To < (Tn << 32);
+                            }
+                            if (acc) {
+                                ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
+                                cand |= 1ull << r;
+                            }
+                        }
+                    }
+                }
+                rowmask = wave_or_u64(cand);
+                if (rowmask == 0) break;
             }
             n_live -= (int)wins;
-            __syncthreads();
-
-            // repair: survivors whose nearest hole was removed re-search
-            for (int r = 0; r < rows; ++r) {
-                const int p = r * kWave + lane;
-                const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
-                const uint32_t nn = st >> 16;
-                const uint32_t nn_safe = nn != kNoHole ? nn : 0u;
-                const bool need = nn != kNoHole && hx[nn_safe] == kDeadHoleX;
-                uint64_t mm = ballot(need);
-                while (mm != 0) {
-                    const int l = __builtin_ctzll(mm);
-                    mm &= mm - 1;
-                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)st, l) & 0xFFFFu;
-                    const float x = ex[o], y = ey[o], z = ez[o];
-                    uint64_t key = ~0ull;
-                    for (int h = lane; h < n_h4; h += kWave) {
-                        const float dx = hx[h] - x, dy = hy[h] - y, dz = hz[h] - z;
-                        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        const uint64_t k = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)h;
-                        key = k < key ? k : key;
-                    }
-                    key = wave_min_u64(key);
-                    const uint32_t d2bits = (uint32_t)(key >> 32);
-                    const bool none = d2bits >= 0x7f800000u;
-                    const uint32_t nn_new = none ? kNoHole : (uint32_t)key;
-                    const float w = none ? 0.0f
-                                         : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
-                    if (lane == 0) {
-                        est[r * kWave + l] = (nn_new << 16) | o;
-                        ew[r * kWave + l] = w;
-                    }
-                }
-            }
-            __syncthreads();
 
             // lazy stable compaction when a 64-slot row can be dropped
             if ((n_live + kWave - 1) / kWave < rows) {
@@ -282,6 +352,12 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     }
     for (int b = bin + lane; b < nb; b += kWave) cnt[b] = 0u;
     if (a.live_steps != nullptr && lane == 0) a.live_steps[c] = lsteps;
+    if (a.work != nullptr && lane == 0) {
+        a.work[4 * (size_t)c] = lsteps;
+        a.work[4 * (size_t)c + 1] = dist_evals;
+        a.work[4 * (size_t)c + 2] = (uint64_t)(n_e0 - n_live);
+        a.work[4 * (size_t)c + 3] = (uint64_t)steps_run;
+    }
 
     if (a.dump.enabled) {
         const StateDump& d = a.dump;

@@ -34,6 +34,7 @@ struct Plan {
     mcl::RunDesc* d_runs = nullptr;
     int32_t* d_crun = nullptr;
     int32_t* d_csim = nullptr;
+    uint64_t* d_work = nullptr;   // [total][4] per-cluster work counters
     uint32_t key0 = 0, key1 = 0;
     int64_t total = 0;
     int32_t n_bins = 0;
@@ -143,6 +144,7 @@ void plan_free(Plan* p)
     hipFree(p->d_runs);
     hipFree(p->d_crun);
     hipFree(p->d_csim);
+    hipFree(p->d_work);
     if (p->ev0) hipEventDestroy(p->ev0);
     if (p->ev1) hipEventDestroy(p->ev1);
     delete p;
@@ -199,6 +201,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     hipError_t e = hipMalloc((void**)&p->d_runs, sizeof(mcl::RunDesc) * (size_t)n_runs);
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_crun, sizeof(int32_t) * (size_t)total);
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_csim, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_work, sizeof(uint64_t) * 4 * (size_t)total);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p->d_runs, desc.data(), sizeof(mcl::RunDesc) * (size_t)n_runs,
                            hipMemcpyHostToDevice, c->stream);
@@ -233,6 +236,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.n_bins = p->n_bins;
     a.counts = counts;
     a.live_steps = live;
+    a.work = p->d_work;
     a.h_cap = p->h_cap;
     a.e_cap = p->e_cap;
     a.dump.enabled = 0;
@@ -311,8 +315,8 @@ int mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_hol
 {
     const Ctx* c = (const Ctx*)ctx;
     if (!c || !max_electrons || !max_holes) return MCL_EINVAL;
-    // equal electron and hole counts: 16 (h + 4) + 20 (e + 64) <= max_lds
-    int n = (c->max_lds - 16 * 4 - 20 * 64) / 36;
+    // equal electron and hole counts: 16 (h + 4) + 24 (e + 64) <= max_lds
+    int n = (c->max_lds - 16 * 4 - 24 * 64) / 40;
     n = (n / 64) * 64;
     *max_electrons = n > 65535 ? 65535 : n;
     *max_holes = n > 65534 ? 65534 : n;
@@ -370,6 +374,29 @@ int mcl_plan_last_ms(void* plan, double* ms)
     float f = 0.0f;
     HIP_TRY(p->ctx, hipEventElapsedTime(&f, p->ev0, p->ev1));
     *ms = (double)f;
+    return MCL_OK;
+}
+
+int mcl_plan_work(void* plan, mcl_work* out)
+{
+    Plan* p = (Plan*)plan;
+    if (!p || !out) return MCL_EINVAL;
+    Ctx* c = p->ctx;
+    if (!p->timed) return fail(c, MCL_EINVAL, "plan has not been launched");
+    std::vector<uint64_t> w(4 * (size_t)p->total);
+    HIP_TRY(c, hipMemcpyAsync(w.data(), p->d_work, sizeof(uint64_t) * w.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::memset(out, 0, sizeof *out);
+    for (size_t i = 0; i < (size_t)p->total; ++i) {
+        out->live_electron_steps += w[4 * i];
+        out->distance_evals += w[4 * i + 1];
+        out->recombinations += w[4 * i + 2];
+        out->cluster_steps += w[4 * i + 3];
+    }
+    float f = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&f, p->ev0, p->ev1));
+    out->kernel_ms = (double)f;
     return MCL_OK;
 }
 
@@ -431,13 +458,12 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
         mcl_plan_last_ms(p, &ms);
         stats->kernel_ms = ms;
         stats->clusters = (uint64_t)p->total;
-        for (uint64_t v : live) stats->live_electron_steps += v;
-        if (!dev) {
-            for (size_t i = 0; i < n_counts; ++i) stats->recombinations += counts[i];
+        mcl_work w;
+        if (mcl_plan_work(p, &w) == MCL_OK) {
+            stats->live_electron_steps = w.live_electron_steps;
+            stats->recombinations = w.recombinations;
+            stats->cluster_steps = w.cluster_steps;
         }
-        int64_t k = 0;
-        for (int r = 0; r < n_runs; ++r)
-            for (int s = 0; s < runs[r].sims; ++s, ++k) stats->cluster_steps += (uint64_t)runs[r].n_steps;
     }
     if (!dev) hipFree(d_counts);
     hipFree(d_live);

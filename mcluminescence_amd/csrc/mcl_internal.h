@@ -44,6 +44,8 @@ struct LaunchArgs {
     int32_t n_bins;
     uint32_t* counts;       // [n_clusters][n_bins]
     uint64_t* live_steps;   // [n_clusters] or null
+    uint64_t* work;         // [n_clusters][4] or null: live electron-steps, distance
+                            // evaluations, recombinations, steps simulated
     int32_t h_cap;          // LDS hole capacity (multiple of 4)
     int32_t e_cap;          // LDS electron capacity (multiple of 64)
     StateDump dump;
@@ -54,7 +56,7 @@ constexpr int kWaveHost = 64;   // wavefront width of gfx950
 // LDS bytes the fixed-dt kernel needs for the given capacities.
 inline int fixed_dt_lds_bytes(int h_cap, int e_cap)
 {
-    return 16 * h_cap + 20 * e_cap;
+    return 16 * h_cap + 24 * e_cap;
 }
 
 hipError_t launch_fixed_dt(const LaunchArgs& a, int lds_bytes, hipStream_t stream);

@@ -12,7 +12,10 @@
  *   rates          engine.py:65-77  (k = s e^{-E_cb/kT} + b e^{-E_loc/kT - alpha r},
  *                                    E_loc = E_loc_2 if U < Retrap else E_loc_1)
  *   step loop      simulate.py:470-516 (start-of-step temperature)
- *   conflicts      src/est_params/functions.py:228-239 (first electron keeps the hole)
+ *   conflicts      src/est_params/functions.py:228-239 (first electron keeps the hole);
+ *                  a loser re-searches its nearest hole and retries within the same
+ *                  step, accepted with probability T_new / T_old (thinning), up to
+ *                  MCL_MAX_ROUNDS resolution rounds per step
  * Random numbers come from Philox4x32-10 (Salmon et al., SC'11) keyed by the
  * run seed and counted by (step block, particle, run, sim|tag); transcendental
  * functions are this file's own Cody-Waite + Taylor evaluations with explicit
@@ -292,20 +295,30 @@ static int cluster_init(cluster_t* c, const mcl_run_params* p, int run, int sim,
 }
 
 /* ------------------------------------------------------------ step rule --- */
-/* Fire threshold for k*dt = x in float32: p = -expm1(-x); fire iff p >= 1 or
- * u32 < (uint32)(p * 2^32). */
-static inline void thr_f32(float x, uint32_t* thr, int* always)
+/* Fire threshold T in [0, 2^32] for x = k dt: p = 1 - e^{-x}; T = 2^32 when
+ * p >= 1, else (uint32)(p * 2^32).  An electron with draw u fires iff u < T. */
+static inline uint64_t thr_f32(float x)
 {
     float p = -orc_expm1f(-x);
-    *always = p >= 1.0f;
-    *thr = *always ? 0xFFFFFFFFu : (uint32_t)(p * 0x1p32f);
+    return p >= 1.0f ? 0x100000000ull : (uint64_t)(uint32_t)(p * 0x1p32f);
 }
 
-static inline void thr_f64(double x, uint64_t* thr, int* always)
+static inline uint64_t thr_f64(double x)
 {
     double p = -expm1(-x);
-    *always = p >= 1.0;
-    *thr = *always ? 0xFFFFFFFFull : (uint64_t)(p * 0x1p32);
+    return p >= 1.0 ? 0x100000000ull : (uint64_t)(p * 0x1p32);
+}
+
+typedef struct {
+    double A, B1, B2;         /* s e^{-E_cb/kT}, b e^{-E_loc_i/kT}           */
+    float xA, b1dt, b2dt;     /* float32 A dt, B_i dt                          */
+} step_scalars_t;
+
+static inline uint64_t electron_thr(const cluster_t* c, const mcl_run_params* p,
+                                    const step_scalars_t* s, int e, int chan2, int fp64)
+{
+    if (fp64) return thr_f64(s->A * p->dt + (chan2 ? s->B2 : s->B1) * p->dt * c->dw[e]);
+    return thr_f32(fmaf(chan2 ? s->b2dt : s->b1dt, c->w[e], s->xA));
 }
 
 static int validate(const mcl_run_params* p, int n_bins)
@@ -347,7 +360,9 @@ static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uin
     const uint32_t c3f = ((uint32_t)sim << 4) | MCL_TAG_FIRE;
     const uint32_t c3c = ((uint32_t)sim << 4) | MCL_TAG_CHAN;
 
-    int32_t* fired = xcalloc(c.n_e0, sizeof(int32_t));
+    int32_t* cand = xcalloc(c.n_e0, sizeof(int32_t));
+    int32_t* lose = xcalloc(c.n_e0, sizeof(int32_t));
+    uint64_t* thr_prev = xcalloc(c.n_e0, sizeof(uint64_t));
     uint32_t* claim = xcalloc(c.n_h, sizeof(uint32_t));
     for (int h = 0; h < c.n_h; ++h) claim[h] = 0xFFFFFFFFu;
     unsigned char* won = xcalloc(c.n_e0, 1);
@@ -372,9 +387,10 @@ static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uin
         }
         const uint32_t blk = (uint32_t)n >> 2;
         const int word = n & 3;
-        const float xA = (float)(A * p->dt);
-        const float b1dt = (float)(B1 * p->dt), b2dt = (float)(B2 * p->dt);
-        int n_fired = 0;
+        step_scalars_t sc = {A, B1, B2, (float)(A * p->dt), (float)(B1 * p->dt),
+                             (float)(B2 * p->dt)};
+        /* round 0 candidates: electrons whose draw falls below their threshold */
+        int n_cand = 0;
         for (int i = 0; i < c.n_live; ++i) {
             const int e = c.live[i];
             if (c.nn[e] == MCL_NO_HOLE) continue;
@@ -384,45 +400,60 @@ static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uin
                 chan2 = (uint64_t)uc < ret_thr;
             }
             const uint32_t u = philox_word(blk, (uint32_t)e, (uint32_t)run, c3f, key, word);
-            int fire;
-            if (fp64) {
-                const double x = A * p->dt + (chan2 ? B2 : B1) * p->dt * c.dw[e];
-                uint64_t thr; int always;
-                thr_f64(x, &thr, &always);
-                fire = always || (uint64_t)u < thr;
-            } else {
-                const float x = fmaf(chan2 ? b2dt : b1dt, c.w[e], xA);
-                uint32_t thr; int always;
-                thr_f32(x, &thr, &always);
-                fire = always || u < thr;
-            }
-            if (fire) fired[n_fired++] = e;   /* ascending electron index */
+            const uint64_t T = electron_thr(&c, p, &sc, e, chan2, fp64);
+            if ((uint64_t)u < T) { cand[n_cand++] = e; thr_prev[e] = T; }   /* ascending */
         }
-        if (n_fired == 0) continue;
-        /* conflict rule: the lowest electron index keeps a shared hole */
+        if (n_cand == 0) continue;
         int wins = 0;
-        for (int j = 0; j < n_fired; ++j) {
-            const int e = fired[j];
-            const uint32_t h = c.nn[e];
-            if (claim[h] == 0xFFFFFFFFu) { claim[h] = (uint32_t)e; won[e] = 1; ++wins; }
-        }
-        for (int j = 0; j < n_fired; ++j) {
-            const int e = fired[j];
-            if (won[e]) { c.h_alive[c.nn[e]] = 0; }
-            claim[c.nn[e]] = 0xFFFFFFFFu;
-        }
-        /* remove winners (stable), re-search electrons whose hole died */
-        int k = 0;
-        for (int i = 0; i < c.n_live; ++i) {
-            const int e = c.live[i];
-            if (won[e]) { won[e] = 0; continue; }
-            c.live[k++] = e;
-        }
-        c.n_live = k;
-        for (int i = 0; i < c.n_live; ++i) {
-            const int e = c.live[i];
-            if (c.nn[e] != MCL_NO_HOLE && !c.h_alive[c.nn[e]])
-                refresh_nn(&c, e, alpha_f, alpha_nm, fp64);
+        for (int round = 0; round < MCL_MAX_ROUNDS && n_cand > 0; ++round) {
+            /* the lowest electron index keeps a shared hole (functions.py:228-239) */
+            int n_lose = 0;
+            for (int j = 0; j < n_cand; ++j) {
+                const int e = cand[j];
+                const uint32_t h = c.nn[e];
+                if (claim[h] == 0xFFFFFFFFu) { claim[h] = (uint32_t)e; won[e] = 1; ++wins; }
+                else lose[n_lose++] = e;
+            }
+            for (int j = 0; j < n_cand; ++j) {
+                const int e = cand[j];
+                if (won[e]) c.h_alive[c.nn[e]] = 0;
+                claim[c.nn[e]] = 0xFFFFFFFFu;
+            }
+            /* remove winners (stable), re-search electrons whose hole died */
+            int k = 0;
+            for (int i = 0; i < c.n_live; ++i) {
+                const int e = c.live[i];
+                if (won[e]) { won[e] = 0; continue; }
+                c.live[k++] = e;
+            }
+            c.n_live = k;
+            for (int i = 0; i < c.n_live; ++i) {
+                const int e = c.live[i];
+                if (c.nn[e] != MCL_NO_HOLE && !c.h_alive[c.nn[e]])
+                    refresh_nn(&c, e, alpha_f, alpha_nm, fp64);
+            }
+            /* a loser retries on its new nearest hole in the same step, accepted
+             * with probability T_new / T_old (thinning of the draw it already won) */
+            n_cand = 0;
+            if (round + 1 == MCL_MAX_ROUNDS) break;
+            const uint32_t c3r = ((uint32_t)sim << 4) | (MCL_TAG_RETRY0 + (uint32_t)round);
+            for (int j = 0; j < n_lose; ++j) {
+                const int e = lose[j];
+                if (c.nn[e] == MCL_NO_HOLE) continue;
+                int chan2 = always_chan2;
+                if (two_channel) {
+                    uint32_t uc = philox_word(blk, (uint32_t)e, (uint32_t)run, c3c, key, word);
+                    chan2 = (uint64_t)uc < ret_thr;
+                }
+                const uint64_t Tn = electron_thr(&c, p, &sc, e, chan2, fp64);
+                const uint64_t To = thr_prev[e];
+                int accept = Tn >= To;
+                if (!accept) {
+                    const uint32_t u2 = philox_word(blk, (uint32_t)e, (uint32_t)run, c3r, key, word);
+                    accept = (uint64_t)u2 * To < (Tn << 32);
+                }
+                if (accept) { cand[n_cand++] = e; thr_prev[e] = Tn; }
+            }
         }
         const int bin = (int)(((int64_t)n * n_bins) / n_steps);
         counts[bin] += (uint32_t)wins;
@@ -441,7 +472,7 @@ static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uin
             }
         }
     }
-    free(fired); free(claim); free(won);
+    free(cand); free(lose); free(thr_prev); free(claim); free(won);
     cluster_free(&c);
     return live_steps;
 }

@@ -85,3 +85,22 @@ def test_state_after_steps_matches_oracle_and_bruteforce(gpu_ctx):
         np.testing.assert_array_equal(g["nn"][live], o["nn"][live])
         bf = orc.bruteforce_nn(o["e_xyz"][live], o["h_xyz"], o["h_alive"])
         np.testing.assert_array_equal(g["nn"][live], bf)
+
+
+@pytest.mark.parametrize("T_K", [540.0, 560.0, 700.0])
+def test_conflict_heavy_iso_bit_exact(gpu_ctx, T_K):
+    # p = 0.35, 0.71 and 1: many shared-hole conflicts and multi-round retries per step
+    runs = W.iso_decay_4096x400(sims=64)
+    runs[0].T_start = T_K - 273.15
+    g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+
+
+def test_dense_tunnelling_retries_bit_exact(gpu_ctx):
+    # high rho': short distances, retries decided by the T_new / T_old thinning draw
+    r = W.tunnelling_tl12(sims=8, n=800, T_rate=5.0)[0]
+    r.rho_prime = 3e-3
+    r2 = W.tunnelling_tl12(sims=8, n=800, T_rate=5.0)[0]
+    r2.rho_prime, r2.E_loc_1, r2.E_loc_2, r2.Retrap = 3e-3, 1.1, 1.3, 0.6
+    g, gl, o, ol, _ = _replay(gpu_ctx, [r, r2])
+    _assert_same(g, gl, o, ol)
