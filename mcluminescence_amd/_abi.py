@@ -134,6 +134,14 @@ def library() -> C.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP library first "
             "(python -m mcluminescence_amd.csrc.build); there is no CPU fallback")
+    # PyTorch-ROCm ships its own libamdhip64 and links it by the unversioned
+    # name.  Loading torch first makes libmcl.so bind to that same runtime (same
+    # SONAME); loading libmcl.so first would let a later torch import start a
+    # second HIP runtime in the process, which then sees no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -4,12 +4,28 @@
 // TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of one
 // wavefront owns one cluster; everything the step touches lives in LDS:
 //
-//   hx, hy, hz [h_cap]   hole coordinates, float32 nm, SoA (removed hole: hx = 3e38)
-//   claim      [h_cap]   per-hole lowest firing electron index (conflict rule)
-//   ex, ey, ez [e_cap]   electron coordinates, indexed by original electron index
-//   est        [e_cap]   compacted live electrons: (nearest hole << 16) | electron index
+//   hx, hy, hz [h_cap]   hole coordinates, float32 nm, sorted by grid cell
+//                        (a removed hole keeps its slot with hx = 3e38)
+//   hmeta      [h_cap]   (claiming electron << 16) | original hole index
+//   cstart     [g_cap]   first sorted slot of each cell, cstart[G^3] = n_h
+//   est        [e_cap]   compacted live electrons: (nearest hole slot << 16) | electron index
 //   ew         [e_cap]   compacted w = exp(-alpha r_nn), float32
 //   ethr       [e_cap]   fire threshold of a candidate in the current step (conflict retries)
+//   rq         [e_cap]   electrons whose nearest hole must be searched again: (slot << 16) | e
+//
+// Electron coordinates are not stored: they are a pure function of the
+// Philox counter (0, e, run, sim|GEOM_E) and are regenerated when an
+// electron's nearest hole has to be searched again.
+//
+// Nearest holes come from a uniform G^3 cell grid over the hole cube: a search
+// scans the 3x3x3 cells around the electron and accepts the minimum of
+// (d2 bits, original hole index) only if it is closer than every hole outside
+// that block can be (guaranteed face distance, with a margin far above float32
+// rounding); otherwise the whole wave scans every hole for that electron.  The
+// answer is therefore always the first-index argmin of the full scan
+// (engine.py:118-119, 173-175); the grid only removes work.  All electrons that
+// need a search in a step are queued and searched lane-parallel, so a step pays
+// the latency of one search, not one per recombination.
 //
 // Per step n (start-of-step temperature T_n = T0 + T_rate n dt, simulate.py:477-479):
 //   scalars   A = s e^{-E_cb/kT}, B_i = b e^{-E_loc_i/kT} in float64 (own exp),
@@ -18,12 +34,12 @@
 //             x = fma(b dt, w, x_A); fire iff u < (uint32)((1 - e^{-x}) 2^32).
 //             When fma(b dt, w, x_A) == x_A (tunnelling below one ulp of the
 //             band term) the per-step threshold is reused: same function of x,
-//             no per-electron transcendental.
-//   conflict  ds_min_u32 of the electron index into claim[nn]; the lowest index
-//             keeps the hole (src/est_params/functions.py:228-239).
+//             no per-electron transcendental.  Two rows per iteration keep two
+//             independent Philox chains in flight.
+//   conflict  ds_min_u32 of the electron index into the hole's claim field; the
+//             lowest index keeps the hole (src/est_params/functions.py:228-239).
 //   remove    winners' slots become dead, their holes get the x sentinel.
-//   repair    survivors whose hole died re-search: one wave scan over the holes,
-//             (d2 bits, index) min-reduced across the wave = first-index argmin.
+//   repair    survivors whose hole died are queued and searched again.
 //   retry     a conflict loser retries on its new nearest hole within the step,
 //             accepted with probability T_new / T_old (one more Philox word), for
 //             up to MCL_MAX_ROUNDS rounds; this removes the O(k dt) delay a
@@ -42,6 +58,143 @@ __device__ __forceinline__ float unit_coord(uint32_t word, double side)
     return (float)(((double)word * 0x1p-32) * side);
 }
 
+__device__ __forceinline__ int cell_of(float v, float inv_cell, int G)
+{
+    const int c = (int)(v * inv_cell);   // v >= 0
+    return c < G ? c : G - 1;
+}
+
+__device__ __forceinline__ uint64_t hole_key(float d2, uint32_t meta, uint32_t slot)
+{
+    // (d2 bits, original index, slot): the min is the first-index argmin of d2
+    return ((uint64_t)__float_as_uint(d2) << 32) | ((meta & 0xFFFFu) << 16) | slot;
+}
+
+__device__ __forceinline__ float sq_dist(float hx, float hy, float hz, float x, float y, float z)
+{
+    const float dx = hx - x, dy = hy - y, dz = hz - z;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Distance from v to the nearest face of cells [lo, hi] that has cells beyond it.
+__device__ __forceinline__ float face_gap(float v, int lo, int hi, int G, float cell)
+{
+    float g = __builtin_inff();
+    if (lo > 0) g = v - (float)lo * cell;
+    if (hi < G - 1) g = fminf(g, (float)(hi + 1) * cell - v);
+    return g;
+}
+
+// Is the block minimum d2 provably below every hole outside the block?
+__device__ __forceinline__ bool block_conclusive(float best_d2, float gap, float cell)
+{
+    const float m = gap - 1e-4f * cell;
+    return m > 0.0f && best_d2 < m * m * (1.0f - 1e-5f);
+}
+
+struct Grid {
+    const float* hx;
+    const float* hy;
+    const float* hz;
+    const uint32_t* hmeta;
+    const uint32_t* cstart;
+    int G;
+    float cell, inv_cell;
+    int n_h;
+};
+
+struct ElectronGen {
+    uint32_t run, sim_tag, k0, k1;
+    double side_nm;
+    __device__ __forceinline__ void xyz(uint32_t e, float& x, float& y, float& z) const
+    {
+        const u32x4 o = philox10(0u, e, run, sim_tag, k0, k1);
+        x = unit_coord(o.x, side_nm);
+        y = unit_coord(o.y, side_nm);
+        z = unit_coord(o.z, side_nm);
+    }
+};
+
+// Lane-private search of the 3x3x3 cell block around (x, y, z).
+__device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, float y, float z,
+                                                      bool& conclusive, uint64_t& evals)
+{
+    const int G = g.G;
+    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
+              cz = cell_of(z, g.inv_cell, G);
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
+    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
+    uint64_t key = ~0ull;
+    for (int zz = z0; zz <= z1; ++zz) {
+        for (int yy = y0; yy <= y1; ++yy) {
+            const int base = (zz * G + yy) * G;
+            int i = (int)g.cstart[base + x0];
+            const int e = (int)g.cstart[base + x1 + 1];
+            evals += (uint64_t)(e - i);
+            for (; i + 1 < e; i += 2) {
+                const uint64_t ka = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z),
+                                             g.hmeta[i], (uint32_t)i);
+                const uint64_t kb = hole_key(sq_dist(g.hx[i + 1], g.hy[i + 1], g.hz[i + 1], x, y, z),
+                                             g.hmeta[i + 1], (uint32_t)(i + 1));
+                const uint64_t k = ka < kb ? ka : kb;
+                key = k < key ? k : key;
+            }
+            if (i < e) {
+                const uint64_t k = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z), g.hmeta[i],
+                                            (uint32_t)i);
+                key = k < key ? k : key;
+            }
+        }
+    }
+    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
+    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
+                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
+    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
+    return key;
+}
+
+// The whole wave scans every hole for one electron (x, y, z wave-uniform).
+__device__ __forceinline__ uint64_t full_scan_wave(const Grid& g, float x, float y, float z,
+                                                   int lane, uint64_t& evals)
+{
+    if (lane == 0) evals += (uint64_t)g.n_h;   // evaluations are summed over lanes
+    uint64_t key = ~0ull;
+    for (int i = lane; i < g.n_h; i += kWave) {
+        const uint64_t k = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z), g.hmeta[i],
+                                    (uint32_t)i);
+        key = k < key ? k : key;
+    }
+    return wave_min_u64(key);
+}
+
+// Nearest live hole for every active lane's electron: block search per lane,
+// then one wave-wide full scan per lane whose block search was inconclusive.
+__device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x, float y, float z,
+                                               int lane, uint64_t& evals)
+{
+    bool ok = true;
+    uint64_t key = ~0ull;
+    if (act) key = block_search_lane(g, x, y, z, ok, evals);
+    uint64_t fb = ballot(act && !ok);
+    while (fb != 0) {
+        const int l = __builtin_ctzll(fb);
+        fb &= fb - 1;
+        const float fx = __shfl(x, l), fy = __shfl(y, l), fz = __shfl(z, l);
+        const uint64_t k = full_scan_wave(g, fx, fy, fz, lane, evals);
+        if (lane == l) key = k;
+    }
+    return key;
+}
+
+__device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_t& slot, float& w)
+{
+    const uint32_t d2bits = (uint32_t)(key >> 32);
+    const bool none = d2bits >= 0x7f800000u;        // no live hole left
+    slot = none ? kNoHole : (uint32_t)(key & 0xFFFFu);
+    w = none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
+}
+
 __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -54,79 +207,93 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     const int n_e0 = uniform(R.n_e0);
     const int n_h = uniform(R.n_h);
     const int n_steps = uniform(R.n_steps);
-    const int h_cap = a.h_cap, e_cap = a.e_cap;
+    const int G = uniform(R.grid);
+    const int n_cells = G * G * G;
+    const int h_cap = a.h_cap, e_cap = a.e_cap, g_cap = a.g_cap;
 
     float* hx = (float*)lds_raw;
     float* hy = hx + h_cap;
     float* hz = hy + h_cap;
-    uint32_t* claim = (uint32_t*)(hz + h_cap);
-    float* ex = (float*)(claim + h_cap);
-    float* ey = ex + e_cap;
-    float* ez = ey + e_cap;
-    uint32_t* est = (uint32_t*)(ez + e_cap);
+    uint32_t* hmeta = (uint32_t*)(hz + h_cap);
+    uint32_t* cstart = hmeta + h_cap;
+    uint32_t* est = cstart + g_cap;
     float* ew = (float*)(est + e_cap);
     uint32_t* ethr = (uint32_t*)(ew + e_cap);
+    uint32_t* rq = ethr + e_cap;
 
     const uint32_t k0 = a.key0, k1 = a.key1;
     const uint32_t prun = (uint32_t)R.run_id;
     const uint32_t psim = (uint32_t)sim << 4;
+    const float cell = R.cell_nm, inv_cell = R.inv_cell;
+    const ElectronGen egen{prun, psim | MCL_TAG_GEOM_E, k0, k1, R.side_nm};
 
-    // ---- geometry (Box.seed, engine.py:124-129) --------------------------------
-    const int n_h4 = (n_h + 3) & ~3;
-    for (int h = lane; h < n_h4; h += kWave) {
-        float x = kDeadHoleX, y = 0.0f, z = 0.0f;
-        if (h < n_h) {
-            const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
-            x = unit_coord(o.x, R.side_b_nm);
-            y = unit_coord(o.y, R.side_b_nm);
-            z = unit_coord(o.z, R.side_b_nm);
-        }
-        hx[h] = x;
-        hy[h] = y;
-        hz[h] = z;
-        claim[h] = 0xFFFFFFFFu;
-    }
-    for (int e = lane; e < n_e0; e += kWave) {
-        const u32x4 o = philox10(0u, (uint32_t)e, prun, psim | MCL_TAG_GEOM_E, k0, k1);
-        ex[e] = unit_coord(o.x, R.side_nm);
-        ey[e] = unit_coord(o.y, R.side_nm);
-        ez[e] = unit_coord(o.z, R.side_nm);
+    // ---- holes (Box.seed, engine.py:124-129), counting-sorted by grid cell ------
+    for (int i = lane; i <= n_cells; i += kWave) cstart[i] = 0u;
+    __syncthreads();
+    for (int h = lane; h < n_h; h += kWave) {
+        const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
+        const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
+                    z = unit_coord(o.z, R.side_b_nm);
+        const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
+                       cell_of(x, inv_cell, G);
+        atomicAdd(&cstart[cc + 1], 1u);
     }
     __syncthreads();
+    {
+        uint32_t carry = 0;
+        for (int base = 0; base <= n_cells; base += kWave) {
+            const int i = base + lane;
+            uint32_t v = i <= n_cells ? cstart[i] : 0u;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t u = (uint32_t)__shfl_up((int)v, off);
+                if (lane >= off) v += u;
+            }
+            v += carry;
+            if (i <= n_cells) cstart[i] = v;
+            carry = (uint32_t)__shfl((int)v, kWave - 1);
+        }
+    }
+    __syncthreads();
+    for (int h = lane; h < n_h; h += kWave) {
+        const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
+        const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
+                    z = unit_coord(o.z, R.side_b_nm);
+        const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
+                       cell_of(x, inv_cell, G);
+        const uint32_t slot = atomicAdd(&cstart[cc], 1u);
+        hx[slot] = x;
+        hy[slot] = y;
+        hz[slot] = z;
+        hmeta[slot] = 0xFFFF0000u | (uint32_t)h;
+    }
+    __syncthreads();
+    // cstart[c] now holds the end of cell c: shift right by one (top chunk first)
+    for (int base = (n_cells / kWave) * kWave; base >= 0; base -= kWave) {
+        const int i = base + lane;
+        const uint32_t v = (i >= 1 && i <= n_cells) ? cstart[i - 1] : 0u;
+        __syncthreads();
+        if (i <= n_cells) cstart[i] = v;
+        __syncthreads();
+    }
 
+    const Grid grid{hx, hy, hz, hmeta, cstart, G, cell, inv_cell, n_h};
     const float alpha_f = (float)R.alpha_nm;
+    uint64_t dist_evals = 0;
 
-    // ---- initial nearest hole (Box._rebuild, engine.py:113-119) -----------------
-    // Lane per electron; every lane reads the same hole (LDS broadcast), four
-    // holes per float4 load, strict < in increasing index = first-index argmin.
+    // ---- electrons and their initial nearest hole (Box._rebuild, engine.py:113-119)
     const int rows0 = (n_e0 + kWave - 1) / kWave;
     for (int r = 0; r < rows0; ++r) {
         const int e = r * kWave + lane;
         const bool act = e < n_e0;
-        const float x = act ? ex[e] : 0.0f;
-        const float y = act ? ey[e] : 0.0f;
-        const float z = act ? ez[e] : 0.0f;
-        float best = __builtin_inff();
-        uint32_t idx = kNoHole;
-        for (int h = 0; h < n_h4; h += 4) {
-            const float4 X = *(const float4*)(hx + h);
-            const float4 Y = *(const float4*)(hy + h);
-            const float4 Z = *(const float4*)(hz + h);
-#define MCL_NN_STEP(cx, cy, cz, j)                                                   \
-            {                                                                        \
-                const float dx = cx - x, dy = cy - y, dz = cz - z;                  \
-                const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)); \
-                if (d2 < best) { best = d2; idx = (uint32_t)(h + j); }               \
-            }
-            MCL_NN_STEP(X.x, Y.x, Z.x, 0)
-            MCL_NN_STEP(X.y, Y.y, Z.y, 1)
-            MCL_NN_STEP(X.z, Y.z, Z.z, 2)
-            MCL_NN_STEP(X.w, Y.w, Z.w, 3)
-#undef MCL_NN_STEP
-        }
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        if (act) egen.xyz((uint32_t)e, x, y, z);
+        const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
         if (act) {
-            const float w = idx == kNoHole ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(best)));
-            est[e] = (idx << 16) | (uint32_t)e;
+            uint32_t slot;
+            float w;
+            nn_from_key(key, alpha_f, slot, w);
+            est[e] = (slot << 16) | (uint32_t)e;
             ew[e] = w;
         }
     }
@@ -140,10 +307,12 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     const uint64_t ret_thr = R.ret_thr;
     const uint32_t c3f = psim | MCL_TAG_FIRE, c3c = psim | MCL_TAG_CHAN;
     const int step_limit = a.dump.enabled ? min(a.dump.max_steps, n_steps) : n_steps;
+    // single-trap parameter sets share one Boltzmann factor between the channels
+    const bool b1_is_a = R.b == R.s && R.E_loc_1 == R.E_cb;
+    const bool b2_is_b1 = R.E_loc_2 == R.E_loc_1;
 
     int n_live = n_e0, n_slots = n_e0;
     uint64_t lsteps = 0;
-    uint64_t dist_evals = (uint64_t)n_e0 * (uint64_t)n_h;
     int steps_run = 0;
     uint32_t bin_acc = 0;
     int bin = 0;
@@ -160,8 +329,8 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
             const double T = R.T0_K + R.T_rate * t;
             const double kT = R.k_b * T;
             const double A = R.s * exp_own(-(R.E_cb / kT));
-            const double B1 = R.b * exp_own(-(R.E_loc_1 / kT));
-            const double B2 = R.b * exp_own(-(R.E_loc_2 / kT));
+            const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
+            const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
             xA = (float)(A * R.dt);
             b1dt = (float)(B1 * R.dt);
             b2dt = (float)(B2 * R.dt);
@@ -171,30 +340,39 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         const int word = n & 3;
         const int rows = (n_slots + kWave - 1) / kWave;
 
-        // round 0: fire test + claims
+        // round 0: fire test + claims, two rows per iteration
         uint64_t cand = 0;
-        for (int r = 0; r < rows; ++r) {
-            const int p = r * kWave + lane;
-            const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
-            const uint32_t o = st & 0xFFFFu, nn = st >> 16;
-            if (nn != kNoHole) {
-                const float w = ew[p];
-                bool chan2 = always2;
-                if (two_ch) {
-                    const u32x4 rc = philox10(blk, o, prun, c3c, k0, k1);
-                    chan2 = (uint64_t)pick_word(rc, word) < ret_thr;
-                }
-                const u32x4 rf = philox10(blk, o, prun, c3f, k0, k1);
-                const uint32_t u = pick_word(rf, word);
-                const float x = __builtin_fmaf(chan2 ? b2dt : b1dt, w, xA);
-                FireThr t = tA;
-                if (x != xA) t = fire_threshold(x);
-                if (t.always || u < t.thr) {
-                    atomicMin(&claim[nn], o);
-                    ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
-                    cand |= 1ull << r;
-                }
+        for (int r = 0; r < rows; r += 2) {
+            const int p0 = r * kWave + lane, p1 = p0 + kWave;
+            const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
+            const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
+            const float w0 = ew[p0], w1 = ew[p1];
+            const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
+            const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);
+            const uint32_t u1 = pick_word(philox10(blk, o1, prun, c3f, k0, k1), word);
+            bool ch0 = always2, ch1 = always2;
+            if (two_ch) {
+                ch0 = (uint64_t)pick_word(philox10(blk, o0, prun, c3c, k0, k1), word) < ret_thr;
+                ch1 = (uint64_t)pick_word(philox10(blk, o1, prun, c3c, k0, k1), word) < ret_thr;
             }
+            const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
+            const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
+#define MCL_FIRE_TAIL(st, o, p, u, x, rr)                                              \
+            {                                                                         \
+                const uint32_t nn = st >> 16;                                         \
+                if (nn != kNoHole) {                                                  \
+                    FireThr t = tA;                                                   \
+                    if (x != xA) t = fire_threshold(x);                               \
+                    if (t.always || u < t.thr) {                                      \
+                        atomicMin(&hmeta[nn], (o << 16) | (hmeta[nn] & 0xFFFFu));    \
+                        ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;                     \
+                        cand |= 1ull << (rr);                                         \
+                    }                                                                 \
+                }                                                                     \
+            }
+            MCL_FIRE_TAIL(st0, o0, p0, u0, x0, r)
+            MCL_FIRE_TAIL(st1, o1, p1, u1, x1, r + 1)
+#undef MCL_FIRE_TAIL
         }
         uint64_t rowmask = wave_or_u64(cand);
         uint32_t wins = 0;
@@ -208,12 +386,13 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                         m &= m - 1;
                         if ((cand >> r) & 1ull) {
                             const uint32_t st = est[r * kWave + lane];
-                            atomicMin(&claim[st >> 16], st & 0xFFFFu);
+                            const uint32_t nn = st >> 16;
+                            atomicMin(&hmeta[nn], ((st & 0xFFFFu) << 16) | (hmeta[nn] & 0xFFFFu));
                         }
                     }
                 }
                 __syncthreads();
-                // winners: claim[nn] holds the lowest candidate index for that hole
+                // winners: the claim field holds the lowest candidate index for that hole
                 uint64_t lose = 0;
                 uint64_t m = rowmask;
                 while (m != 0) {
@@ -222,15 +401,16 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                     const int p = r * kWave + lane;
                     const bool is_cand = (cand >> r) & 1ull;
                     bool win = false;
-                    uint32_t nn = 0;
+                    uint32_t nn = 0, meta = 0;
                     if (is_cand) {
                         const uint32_t st = est[p];
                         nn = st >> 16;
-                        win = claim[nn] == (st & 0xFFFFu);
+                        meta = hmeta[nn];
+                        win = (meta >> 16) == (st & 0xFFFFu);
                         if (!win) lose |= 1ull << r;
                     }
                     if (win) {
-                        claim[nn] = 0xFFFFFFFFu;
+                        hmeta[nn] = 0xFFFF0000u | (meta & 0xFFFFu);
                         hx[nn] = kDeadHoleX;
                         est[p] = kDeadSlot;
                     }
@@ -238,37 +418,34 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                 }
                 __syncthreads();
 
-                // repair: survivors whose nearest hole was removed re-search
+                // repair: queue the survivors whose nearest hole was removed ...
+                int nq = 0;
                 for (int r = 0; r < rows; ++r) {
                     const int p = r * kWave + lane;
                     const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
                     const uint32_t nn = st >> 16;
                     const uint32_t nn_safe = nn != kNoHole ? nn : 0u;
                     const bool need = nn != kNoHole && hx[nn_safe] == kDeadHoleX;
-                    uint64_t mm = ballot(need);
-                    while (mm != 0) {
-                        const int l = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-                        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)st, l) & 0xFFFFu;
-                        const float x = ex[o], y = ey[o], z = ez[o];
-                        uint64_t key = ~0ull;
-                        for (int h = lane; h < n_h4; h += kWave) {
-                            const float dx = hx[h] - x, dy = hy[h] - y, dz = hz[h] - z;
-                            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                            const uint64_t k = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)h;
-                            key = k < key ? k : key;
-                        }
-                        key = wave_min_u64(key);
-                        dist_evals += (uint64_t)n_h;
-                        const uint32_t d2bits = (uint32_t)(key >> 32);
-                        const bool none = d2bits >= 0x7f800000u;
-                        const uint32_t nn_new = none ? kNoHole : (uint32_t)key;
-                        const float w =
-                            none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
-                        if (lane == 0) {
-                            est[r * kWave + l] = (nn_new << 16) | o;
-                            ew[r * kWave + l] = w;
-                        }
+                    const uint64_t mm = ballot(need);
+                    if (need) rq[nq + lanes_below(mm)] = ((uint32_t)p << 16) | (st & 0xFFFFu);
+                    nq += __popcll(mm);
+                }
+                __syncthreads();
+                // ... and search them lane-parallel
+                for (int q0 = 0; q0 < nq; q0 += kWave) {
+                    const int i = q0 + lane;
+                    const bool act = i < nq;
+                    const uint32_t q = act ? rq[i] : 0u;
+                    const uint32_t o = q & 0xFFFFu;
+                    float x = 0.0f, y = 0.0f, z = 0.0f;
+                    if (act) egen.xyz(o, x, y, z);
+                    const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
+                    if (act) {
+                        uint32_t slot;
+                        float w;
+                        nn_from_key(key, alpha_f, slot, w);
+                        est[q >> 16] = (slot << 16) | o;
+                        ew[q >> 16] = w;
                     }
                 }
                 __syncthreads();
@@ -352,9 +529,17 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     }
     for (int b = bin + lane; b < nb; b += kWave) cnt[b] = 0u;
     if (a.live_steps != nullptr && lane == 0) a.live_steps[c] = lsteps;
+    // lanes counted their own evaluations in the lane-private searches
+    uint64_t evals_total = dist_evals;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)evals_total, off);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(evals_total >> 32), off);
+        evals_total += ((uint64_t)hi << 32) | lo;
+    }
     if (a.work != nullptr && lane == 0) {
         a.work[4 * (size_t)c] = lsteps;
-        a.work[4 * (size_t)c + 1] = dist_evals;
+        a.work[4 * (size_t)c + 1] = evals_total;
         a.work[4 * (size_t)c + 2] = (uint64_t)(n_e0 - n_live);
         a.work[4 * (size_t)c + 3] = (uint64_t)steps_run;
     }
@@ -363,28 +548,32 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         const StateDump& d = a.dump;
         __syncthreads();
         for (int e = lane; e < n_e0; e += kWave) {
+            float x, y, z;
+            egen.xyz((uint32_t)e, x, y, z);
             d.e_live[e] = 0;
             d.nn[e] = kNoHole;
             d.w[e] = 0.0f;
-            d.e_xyz[3 * e] = ex[e];
-            d.e_xyz[3 * e + 1] = ey[e];
-            d.e_xyz[3 * e + 2] = ez[e];
+            d.e_xyz[3 * e] = x;
+            d.e_xyz[3 * e + 1] = y;
+            d.e_xyz[3 * e + 2] = z;
         }
         __syncthreads();
         for (int p = lane; p < n_slots; p += kWave) {
             const uint32_t st = est[p];
-            if (st != kDeadSlot) {
-                const uint32_t o = st & 0xFFFFu;
+            const uint32_t o = st & 0xFFFFu, slot = st >> 16;
+            if (st != kDeadSlot && o < (uint32_t)n_e0) {
                 d.e_live[o] = 1;
-                d.nn[o] = st >> 16;
+                d.nn[o] = slot == kNoHole ? kNoHole : (hmeta[slot] & 0xFFFFu);
                 d.w[o] = ew[p];
             }
         }
-        for (int h = lane; h < n_h; h += kWave) {
-            d.h_alive[h] = hx[h] != kDeadHoleX;
-            d.h_xyz[3 * h] = hx[h];
-            d.h_xyz[3 * h + 1] = hy[h];
-            d.h_xyz[3 * h + 2] = hz[h];
+        for (int s = lane; s < n_h; s += kWave) {
+            const uint32_t h = hmeta[s] & 0xFFFFu;
+            if (h >= (uint32_t)n_h) continue;
+            d.h_alive[h] = hx[s] != kDeadHoleX;
+            d.h_xyz[3 * h] = hx[s];
+            d.h_xyz[3 * h + 1] = hy[s];
+            d.h_xyz[3 * h + 2] = hz[s];
         }
     }
 }

@@ -26,6 +26,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int max_lds = 0;        // dynamic LDS bytes one workgroup may use
     int n_cu = 0;
+    double holes_per_cell = 2.0;   // hole-grid density target (tunable)
     std::string err;
 };
 
@@ -38,7 +39,7 @@ struct Plan {
     uint32_t key0 = 0, key1 = 0;
     int64_t total = 0;
     int32_t n_bins = 0;
-    int32_t h_cap = 0, e_cap = 0, lds = 0;
+    int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0;
     int32_t max_e = 0, max_h = 0;
     uint32_t flags = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -113,7 +114,8 @@ int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geo
     return MCL_OK;
 }
 
-mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id)
+mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id,
+                       double holes_per_cell)
 {
     mcl::RunDesc d{};
     d.side_nm = g.side_m * 1e9;
@@ -135,6 +137,9 @@ mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_i
     d.n_h = g.n_h;
     d.n_steps = p.n_steps;
     d.run_id = run_id;
+    d.grid = mcl::grid_dim(g.n_h, holes_per_cell);
+    d.cell_nm = (float)(d.side_b_nm / d.grid);
+    d.inv_cell = (float)(d.grid / d.side_b_nm);
     return d;
 }
 
@@ -157,23 +162,26 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (!runs || n_runs <= 0) return fail(c, MCL_EINVAL, "no runs");
     std::vector<mcl::RunDesc> desc((size_t)n_runs);
     int64_t total = 0;
-    int max_e = 0, max_h = 0;
+    int max_e = 0, max_h = 0, max_cells = 1;
     for (int r = 0; r < n_runs; ++r) {
         mcl_geometry g;
         int st = validate_run(c, runs[r], r, n_bins, g);
         if (st) return st;
-        desc[(size_t)r] = make_desc(runs[r], g, run_ids ? run_ids[r] : r);
+        desc[(size_t)r] = make_desc(runs[r], g, run_ids ? run_ids[r] : r, c->holes_per_cell);
         total += runs[r].sims;
         if (runs[r].sims > 0) {
             if (g.n_e0 > max_e) max_e = g.n_e0;
             if (g.n_h > max_h) max_h = g.n_h;
+            const int G = desc[(size_t)r].grid;
+            if (G * G * G > max_cells) max_cells = G * G * G;
         }
     }
     if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
     if (total > 0x7FFFFFFF) return fail(c, MCL_EINVAL, "too many clusters (%lld)", (long long)total);
     const int h_cap = ((max_h + 3) / 4) * 4 + 4;
     const int e_cap = ((max_e + 63) / 64) * 64 + 64;
-    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap);
+    const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
+    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
     if (lds > c->max_lds)
         return fail(c, MCL_ERANGE,
                     "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B",
@@ -194,6 +202,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->n_bins = n_bins;
     p->h_cap = h_cap;
     p->e_cap = e_cap;
+    p->g_cap = g_cap;
     p->lds = lds;
     p->max_e = max_e;
     p->max_h = max_h;
@@ -239,6 +248,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.work = p->d_work;
     a.h_cap = p->h_cap;
     a.e_cap = p->e_cap;
+    a.g_cap = p->g_cap;
     a.dump.enabled = 0;
     return a;
 }
@@ -315,8 +325,9 @@ int mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_hol
 {
     const Ctx* c = (const Ctx*)ctx;
     if (!c || !max_electrons || !max_holes) return MCL_EINVAL;
-    // equal electron and hole counts: 16 (h + 4) + 24 (e + 64) <= max_lds
-    int n = (c->max_lds - 16 * 4 - 24 * 64) / 40;
+    // equal electron and hole counts, about 2 holes per grid cell:
+    // 16 (h + 4) + 4 (h / 2 + 4) + 16 (e + 64) <= max_lds
+    int n = (c->max_lds - 16 * 4 - 4 * 4 - 16 * 64) / 34;
     n = (n / 64) * 64;
     *max_electrons = n > 65535 ? 65535 : n;
     *max_holes = n > 65534 ? 65534 : n;

@@ -21,6 +21,10 @@ struct RunDesc {
     int32_t two_channel;   // draw the localisation channel per electron-step
     int32_t always_chan2;  // Retrap >= 1: every draw picks E_loc_2
     int32_t run_id;        // Philox counter word 2 (the run's index in its sweep)
+    int32_t grid;          // G: cells per axis of the hole grid (G^3 cells)
+    float cell_nm;         // cell edge = side_b_nm / G
+    float inv_cell;        // G / side_b_nm
+    int32_t pad_;
 };
 
 // Optional dump of one cluster's kernel state (tests / inspection).
@@ -48,15 +52,25 @@ struct LaunchArgs {
                             // evaluations, recombinations, steps simulated
     int32_t h_cap;          // LDS hole capacity (multiple of 4)
     int32_t e_cap;          // LDS electron capacity (multiple of 64)
+    int32_t g_cap;          // LDS cell-start capacity (>= G^3 + 1, multiple of 4)
     StateDump dump;
 };
 
 constexpr int kWaveHost = 64;   // wavefront width of gfx950
 
 // LDS bytes the fixed-dt kernel needs for the given capacities.
-inline int fixed_dt_lds_bytes(int h_cap, int e_cap)
+inline int fixed_dt_lds_bytes(int h_cap, int e_cap, int g_cap)
 {
-    return 16 * h_cap + 24 * e_cap;
+    return 16 * h_cap + 4 * g_cap + 16 * e_cap;
+}
+
+// Cells per axis of a cluster's hole grid: about holes_per_cell holes a cell.
+inline int grid_dim(int n_h, double holes_per_cell)
+{
+    if (n_h <= 0) return 1;
+    int G = 1;
+    while ((double)(G + 1) * (G + 1) * (G + 1) * holes_per_cell <= (double)n_h && G < 31) ++G;
+    return G;
 }
 
 hipError_t launch_fixed_dt(const LaunchArgs& a, int lds_bytes, hipStream_t stream);

@@ -104,3 +104,36 @@ def test_dense_tunnelling_retries_bit_exact(gpu_ctx):
     r2.rho_prime, r2.E_loc_1, r2.E_loc_2, r2.Retrap = 3e-3, 1.1, 1.3, 0.6
     g, gl, o, ol, _ = _replay(gpu_ctx, [r, r2])
     _assert_same(g, gl, o, ol)
+
+
+def test_grid_edge_cases_bit_exact(gpu_ctx):
+    runs = []
+    # more electrons than holes: holes run out, electrons end with no partner
+    r = W.single_trap(1.0, 1e16, 4e9, 1e23, 120, T_start_K=600.0, T_rate=0.0, dt=1.0,
+                      n_steps=50, sims=3)
+    r.N_e = 400.0
+    r.e_ratio_start = 1.0
+    runs.append(r)
+    # holes confined to a smaller cube than the electrons (bf < 1): searches from
+    # outside the hole grid fall back to the full scan
+    r = W.tunnelling_tl12(sims=3, n=300, T_rate=5.0)[0]
+    r.boundary_factor = 0.6
+    r.n_steps = 50
+    runs.append(r)
+    # a handful of holes: one grid cell
+    r = W.single_trap(1.3, 1e12, 4e9, 1e23, 6, T_start_K=300.0, T_rate=1.0, dt=0.1,
+                      n_steps=50, sims=4)
+    runs.append(r)
+    # sparse, strongly tunnelling cluster at low density
+    r = W.tunnelling_tl12(sims=3, n=500, T_rate=5.0)[0]
+    r.rho_prime, r.n_steps = 3e-6, 50
+    runs.append(r)
+    g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+    # the nearest holes maintained under each of these geometries
+    for run in runs:
+        st = gpu_ctx.cluster_state(run, 0, 1, max_steps=7)
+        ref = orc.final_state(run, 0, 1, 7)
+        np.testing.assert_array_equal(st["e_live"], ref["e_live"])
+        live = st["e_live"].astype(bool)
+        np.testing.assert_array_equal(st["nn"][live], ref["nn"][live])
