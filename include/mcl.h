@@ -106,6 +106,13 @@ const char* mcl_last_error(const void* ctx);
 int         mcl_set_stream(void* ctx, void* hip_stream);
 /* Largest cluster (electrons, holes) one workgroup can hold in LDS. */
 int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_holes);
+/* Kernel tuning knobs of a context, applied to plans made afterwards.  Keys:
+ *   "holes_per_cell"  mean holes per cell of each cluster's nearest-hole grid
+ *                     (default 2; results do not depend on it, only speed).
+ * No counterpart in the reference (it has no kernels); the autotuner
+ * (mcluminescence_amd/autotune.py) sets them. */
+int         mcl_tune_set(void* ctx, const char* key, double value);
+int         mcl_tune_get(const void* ctx, const char* key, double* value);
 
 /* ---- geometry (host only) ------------------------------------------------ */
 /* Replaces the geometry part of TLTrapSim.__init__ (tl_trap_lab.py:33-39). */

@@ -105,6 +105,8 @@ _SIGNATURES = {
     "mcl_last_error": (C.c_char_p, [_P]),
     "mcl_set_stream": (C.c_int, [_P, _P]),
     "mcl_cluster_limits": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mcl_tune_set": (C.c_int, [_P, C.c_char_p, C.c_double]),
+    "mcl_tune_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double)]),
     "mcl_run_geometry": (C.c_int, [C.POINTER(RunParams), C.POINTER(Geometry)]),
     "mcl_run_fixed_dt": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_uint64, C.c_int32,
                                    C.c_uint32, _P, _P, C.POINTER(Stats)]),
@@ -201,6 +203,14 @@ class Context:
 
     def set_stream(self, stream_handle: Optional[int]) -> None:
         self.check(self.lib.mcl_set_stream(self.handle, stream_handle or None), "mcl_set_stream")
+
+    def tune_set(self, key: str, value: float) -> None:
+        self.check(self.lib.mcl_tune_set(self.handle, key.encode(), float(value)), "mcl_tune_set")
+
+    def tune_get(self, key: str) -> float:
+        v = C.c_double()
+        self.check(self.lib.mcl_tune_get(self.handle, key.encode(), C.byref(v)), "mcl_tune_get")
+        return v.value
 
     def cluster_limits(self):
         e, h = C.c_int32(), C.c_int32()

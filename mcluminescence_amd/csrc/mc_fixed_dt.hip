@@ -4,9 +4,9 @@
 // TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of one
 // wavefront owns one cluster; everything the step touches lives in LDS:
 //
-//   hx, hy, hz [h_cap]   hole coordinates, float32 nm, sorted by grid cell
-//                        (a removed hole keeps its slot with hx = 3e38)
-//   hmeta      [h_cap]   (claiming electron << 16) | original hole index
+//   hole       [h_cap]   16-byte records {x, y, z, meta}, float32 nm, sorted by
+//                        grid cell; meta = (claiming electron << 16) | original
+//                        hole index; a removed hole keeps its slot with x = 3e38
 //   cstart     [g_cap]   first sorted slot of each cell, cstart[G^3] = n_h
 //   est        [e_cap]   compacted live electrons: (nearest hole slot << 16) | electron index
 //   ew         [e_cap]   compacted w = exp(-alpha r_nn), float32
@@ -25,7 +25,9 @@
 // answer is therefore always the first-index argmin of the full scan
 // (engine.py:118-119, 173-175); the grid only removes work.  All electrons that
 // need a search in a step are queued and searched lane-parallel, so a step pays
-// the latency of one search, not one per recombination.
+// the latency of one search, not one per recombination; each lane fetches the
+// holes of a cell range eight 16-byte records at a time, so one LDS latency
+// covers a chunk.
 //
 // Per step n (start-of-step temperature T_n = T0 + T_rate n dt, simulate.py:477-479):
 //   scalars   A = s e^{-E_cb/kT}, B_i = b e^{-E_loc_i/kT} in float64 (own exp),
@@ -39,7 +41,8 @@
 //   conflict  ds_min_u32 of the electron index into the hole's claim field; the
 //             lowest index keeps the hole (src/est_params/functions.py:228-239).
 //   remove    winners' slots become dead, their holes get the x sentinel.
-//   repair    survivors whose hole died are queued and searched again.
+//   repair    an electron whose hole was removed is searched again at the start
+//             of the next step, before its fire test (a conflict loser at once).
 //   retry     a conflict loser retries on its new nearest hole within the step,
 //             accepted with probability T_new / T_old (one more Philox word), for
 //             up to MCL_MAX_ROUNDS rounds; this removes the O(k dt) delay a
@@ -70,10 +73,15 @@ __device__ __forceinline__ uint64_t hole_key(float d2, uint32_t meta, uint32_t s
     return ((uint64_t)__float_as_uint(d2) << 32) | ((meta & 0xFFFFu) << 16) | slot;
 }
 
-__device__ __forceinline__ float sq_dist(float hx, float hy, float hz, float x, float y, float z)
+__device__ __forceinline__ float sq_dist(const float4& h, float x, float y, float z)
 {
-    const float dx = hx - x, dy = hy - y, dz = hz - z;
+    const float dx = h.x - x, dy = h.y - y, dz = h.z - z;
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ uint64_t key_of(const float4& h, float x, float y, float z, int slot)
+{
+    return hole_key(sq_dist(h, x, y, z), __float_as_uint(h.w), (uint32_t)slot);
 }
 
 // Distance from v to the nearest face of cells [lo, hi] that has cells beyond it.
@@ -93,10 +101,7 @@ __device__ __forceinline__ bool block_conclusive(float best_d2, float gap, float
 }
 
 struct Grid {
-    const float* hx;
-    const float* hy;
-    const float* hz;
-    const uint32_t* hmeta;
+    const float4* hole;
     const uint32_t* cstart;
     int G;
     float cell, inv_cell;
@@ -115,38 +120,42 @@ struct ElectronGen {
     }
 };
 
-// Lane-private search of the 3x3x3 cell block around (x, y, z).
+constexpr int kChunk = 8;   // hole records fetched per LDS round trip in a search
+
+// Lane-private search of the (2 RAD + 1)^3 cell block around (x, y, z).
+template <int RAD>
 __device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, float y, float z,
                                                       bool& conclusive, uint64_t& evals)
 {
+    constexpr int W = 2 * RAD + 1;
     const int G = g.G;
     const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
               cz = cell_of(z, g.inv_cell, G);
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
-    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
-    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
+    const int x0 = max(cx - RAD, 0), x1 = min(cx + RAD, G - 1);
     uint64_t key = ~0ull;
-    for (int zz = z0; zz <= z1; ++zz) {
-        for (int yy = y0; yy <= y1; ++yy) {
+#pragma unroll
+    for (int k = 0; k < W * W; ++k) {
+        const int zz = cz + k / W - RAD, yy = cy + k % W - RAD;
+        int s = 0, e = 0;
+        if ((unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
             const int base = (zz * G + yy) * G;
-            int i = (int)g.cstart[base + x0];
-            const int e = (int)g.cstart[base + x1 + 1];
-            evals += (uint64_t)(e - i);
-            for (; i + 1 < e; i += 2) {
-                const uint64_t ka = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z),
-                                             g.hmeta[i], (uint32_t)i);
-                const uint64_t kb = hole_key(sq_dist(g.hx[i + 1], g.hy[i + 1], g.hz[i + 1], x, y, z),
-                                             g.hmeta[i + 1], (uint32_t)(i + 1));
-                const uint64_t k = ka < kb ? ka : kb;
-                key = k < key ? k : key;
-            }
-            if (i < e) {
-                const uint64_t k = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z), g.hmeta[i],
-                                            (uint32_t)i);
-                key = k < key ? k : key;
+            s = (int)g.cstart[base + x0];
+            e = (int)g.cstart[base + x1 + 1];
+        }
+        evals += (uint64_t)(e - s);
+        for (int i0 = s; i0 < e; i0 += kChunk) {
+            float4 hv[kChunk];
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t) hv[t] = g.hole[min(i0 + t, e - 1)];
+#pragma unroll
+            for (int t = 0; t < kChunk; ++t) {
+                const uint64_t kk = key_of(hv[t], x, y, z, min(i0 + t, e - 1));
+                key = kk < key ? kk : key;
             }
         }
     }
+    const int y0 = max(cy - RAD, 0), y1 = min(cy + RAD, G - 1);
+    const int z0 = max(cz - RAD, 0), z1 = min(cz + RAD, G - 1);
     const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
     const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
                             fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
@@ -161,21 +170,22 @@ __device__ __forceinline__ uint64_t full_scan_wave(const Grid& g, float x, float
     if (lane == 0) evals += (uint64_t)g.n_h;   // evaluations are summed over lanes
     uint64_t key = ~0ull;
     for (int i = lane; i < g.n_h; i += kWave) {
-        const uint64_t k = hole_key(sq_dist(g.hx[i], g.hy[i], g.hz[i], x, y, z), g.hmeta[i],
-                                    (uint32_t)i);
+        const uint64_t k = key_of(g.hole[i], x, y, z, i);
         key = k < key ? k : key;
     }
     return wave_min_u64(key);
 }
 
-// Nearest live hole for every active lane's electron: block search per lane,
-// then one wave-wide full scan per lane whose block search was inconclusive.
+// Nearest live hole for every active lane's electron: 3x3x3 block search per
+// lane, then one wave-wide full scan per lane whose block was inconclusive.  (A
+// lane-private 5x5x5 level in between measured slower: it is divergent, so one
+// open lane makes the wave pay 25 serialized LDS round trips.)
 __device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x, float y, float z,
                                                int lane, uint64_t& evals)
 {
     bool ok = true;
     uint64_t key = ~0ull;
-    if (act) key = block_search_lane(g, x, y, z, ok, evals);
+    if (act) key = block_search_lane<1>(g, x, y, z, ok, evals);
     uint64_t fb = ballot(act && !ok);
     while (fb != 0) {
         const int l = __builtin_ctzll(fb);
@@ -195,6 +205,15 @@ __device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_
     w = none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
 }
 
+// Bit r set iff any lane has bit r of its row mask set (rows <= 64), by ballots.
+__device__ __forceinline__ uint64_t rows_any(uint64_t bits, int rows)
+{
+    uint64_t m = 0;
+    for (int r = 0; r < rows; ++r)
+        if (ballot((bits >> r) & 1ull) != 0) m |= 1ull << r;
+    return m;
+}
+
 __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -211,11 +230,10 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     const int n_cells = G * G * G;
     const int h_cap = a.h_cap, e_cap = a.e_cap, g_cap = a.g_cap;
 
-    float* hx = (float*)lds_raw;
-    float* hy = hx + h_cap;
-    float* hz = hy + h_cap;
-    uint32_t* hmeta = (uint32_t*)(hz + h_cap);
-    uint32_t* cstart = hmeta + h_cap;
+    float4* hole = (float4*)lds_raw;
+    float* hole_f = (float*)lds_raw;                 // word view: x at 4 s, meta at 4 s + 3
+    uint32_t* hole_u = (uint32_t*)lds_raw;
+    uint32_t* cstart = (uint32_t*)(hole + h_cap);
     uint32_t* est = cstart + g_cap;
     float* ew = (float*)(est + e_cap);
     uint32_t* ethr = (uint32_t*)(ew + e_cap);
@@ -262,10 +280,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
                        cell_of(x, inv_cell, G);
         const uint32_t slot = atomicAdd(&cstart[cc], 1u);
-        hx[slot] = x;
-        hy[slot] = y;
-        hz[slot] = z;
-        hmeta[slot] = 0xFFFF0000u | (uint32_t)h;
+        hole[slot] = make_float4(x, y, z, __uint_as_float(0xFFFF0000u | (uint32_t)h));
     }
     __syncthreads();
     // cstart[c] now holds the end of cell c: shift right by one (top chunk first)
@@ -277,7 +292,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         __syncthreads();
     }
 
-    const Grid grid{hx, hy, hz, hmeta, cstart, G, cell, inv_cell, n_h};
+    const Grid grid{hole, cstart, G, cell, inv_cell, n_h};
     const float alpha_f = (float)R.alpha_nm;
     uint64_t dist_evals = 0;
 
@@ -320,6 +335,47 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     float xA = 0.0f, b1dt = 0.0f, b2dt = 0.0f;
     FireThr tA{0u, false};
 
+    auto hole_dead = [&](uint32_t nn) {
+        return nn != kNoHole && hole_f[4 * (nn != kNoHole ? nn : 0u)] == kDeadHoleX;
+    };
+    // Claim hole nn for electron o: the claim field keeps the lowest index.
+    auto claim = [&](uint32_t nn, uint32_t o) {
+        atomicMin(&hole_u[4 * nn + 3], (o << 16) | (hole_u[4 * nn + 3] & 0xFFFFu));
+    };
+    // Test of one live electron (slot p) with draw u and x = k dt; candidates claim
+    // their hole and remember their threshold for a conflict retry.
+    auto fire_tail = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
+                         uint64_t& cand) {
+        const uint32_t nn = st >> 16;
+        if (nn == kNoHole) return;
+        FireThr t = tA;
+        if (x != xA) t = fire_threshold(x);
+        if (t.always || u < t.thr) {
+            claim(nn, o);
+            ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
+            cand |= 1ull << rr;
+        }
+    };
+    // Search again for the queued electrons rq[0..nq), lane-parallel.
+    auto search_queue = [&](int nq) {
+        for (int q0 = 0; q0 < nq; q0 += kWave) {
+            const int i = q0 + lane;
+            const bool act = i < nq;
+            const uint32_t q = act ? rq[i] : 0u;
+            const uint32_t o = q & 0xFFFFu;
+            float x = 0.0f, y = 0.0f, z = 0.0f;
+            if (act) egen.xyz(o, x, y, z);
+            const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
+            if (act) {
+                uint32_t slot;
+                float w;
+                nn_from_key(key, alpha_f, slot, w);
+                est[q >> 16] = (slot << 16) | o;
+                ew[q >> 16] = w;
+            }
+        }
+    };
+
     for (int n = 0; n < step_limit; ++n) {
         if (n_live == 0) break;
         lsteps += (uint64_t)n_live;
@@ -340,13 +396,18 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         const int word = n & 3;
         const int rows = (n_slots + kWave - 1) / kWave;
 
-        // round 0: fire test + claims, two rows per iteration
-        uint64_t cand = 0;
+        // Pass A, two rows per iteration: an electron whose nearest hole was removed
+        // in the previous step is queued for a new search (its test waits for it);
+        // every other live electron takes its fire test now.  Deferring that search
+        // to here changes nothing: no hole disappears in between.
+        uint64_t cand = 0, stale = 0;
+        int nq = 0;
         for (int r = 0; r < rows; r += 2) {
             const int p0 = r * kWave + lane, p1 = p0 + kWave;
             const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
             const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
             const float w0 = ew[p0], w1 = ew[p1];
+            const bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);
             const uint32_t u1 = pick_word(philox10(blk, o1, prun, c3f, k0, k1), word);
@@ -355,26 +416,37 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                 ch0 = (uint64_t)pick_word(philox10(blk, o0, prun, c3c, k0, k1), word) < ret_thr;
                 ch1 = (uint64_t)pick_word(philox10(blk, o1, prun, c3c, k0, k1), word) < ret_thr;
             }
-            const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
-            const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
-#define MCL_FIRE_TAIL(st, o, p, u, x, rr)                                              \
-            {                                                                         \
-                const uint32_t nn = st >> 16;                                         \
-                if (nn != kNoHole) {                                                  \
-                    FireThr t = tA;                                                   \
-                    if (x != xA) t = fire_threshold(x);                               \
-                    if (t.always || u < t.thr) {                                      \
-                        atomicMin(&hmeta[nn], (o << 16) | (hmeta[nn] & 0xFFFFu));    \
-                        ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;                     \
-                        cand |= 1ull << (rr);                                         \
-                    }                                                                 \
-                }                                                                     \
-            }
-            MCL_FIRE_TAIL(st0, o0, p0, u0, x0, r)
-            MCL_FIRE_TAIL(st1, o1, p1, u1, x1, r + 1)
-#undef MCL_FIRE_TAIL
+            if (!s0) fire_tail(st0, o0, p0, u0, __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA), r, cand);
+            if (!s1) fire_tail(st1, o1, p1, u1, __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA), r + 1, cand);
+            const uint64_t m0 = ballot(s0);
+            if (s0) rq[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
+            nq += __popcll(m0);
+            const uint64_t m1 = ballot(s1);
+            if (s1) rq[nq + lanes_below(m1)] = ((uint32_t)p1 << 16) | o1;
+            nq += __popcll(m1);
+            stale |= ((uint64_t)s0 << r) | ((uint64_t)s1 << (r + 1));
         }
-        uint64_t rowmask = wave_or_u64(cand);
+        if (nq > 0) {
+            __syncthreads();
+            search_queue(nq);
+            __syncthreads();
+            uint64_t m = rows_any(stale, rows);
+            while (m != 0) {
+                const int r = __builtin_ctzll(m);
+                m &= m - 1;
+                const int p = r * kWave + lane;
+                if ((stale >> r) & 1ull) {
+                    const uint32_t st = est[p];
+                    const uint32_t o = st & 0xFFFFu;
+                    const uint32_t u = pick_word(philox10(blk, o, prun, c3f, k0, k1), word);
+                    bool ch = always2;
+                    if (two_ch)
+                        ch = (uint64_t)pick_word(philox10(blk, o, prun, c3c, k0, k1), word) < ret_thr;
+                    fire_tail(st, o, p, u, __builtin_fmaf(ch ? b2dt : b1dt, ew[p], xA), r, cand);
+                }
+            }
+        }
+        uint64_t rowmask = rows_any(cand, rows);
         uint32_t wins = 0;
         if (rowmask != 0) {
             for (int round = 0;; ++round) {
@@ -386,8 +458,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                         m &= m - 1;
                         if ((cand >> r) & 1ull) {
                             const uint32_t st = est[r * kWave + lane];
-                            const uint32_t nn = st >> 16;
-                            atomicMin(&hmeta[nn], ((st & 0xFFFFu) << 16) | (hmeta[nn] & 0xFFFFu));
+                            claim(st >> 16, st & 0xFFFFu);
                         }
                     }
                 }
@@ -405,59 +476,43 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                     if (is_cand) {
                         const uint32_t st = est[p];
                         nn = st >> 16;
-                        meta = hmeta[nn];
+                        meta = hole_u[4 * nn + 3];
                         win = (meta >> 16) == (st & 0xFFFFu);
                         if (!win) lose |= 1ull << r;
                     }
                     if (win) {
-                        hmeta[nn] = 0xFFFF0000u | (meta & 0xFFFFu);
-                        hx[nn] = kDeadHoleX;
+                        hole_u[4 * nn + 3] = 0xFFFF0000u | (meta & 0xFFFFu);
+                        hole_f[4 * nn] = kDeadHoleX;
                         est[p] = kDeadSlot;
                     }
                     wins += (uint32_t)__popcll(ballot(win));
                 }
                 __syncthreads();
-
-                // repair: queue the survivors whose nearest hole was removed ...
-                int nq = 0;
-                for (int r = 0; r < rows; ++r) {
-                    const int p = r * kWave + lane;
-                    const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
-                    const uint32_t nn = st >> 16;
-                    const uint32_t nn_safe = nn != kNoHole ? nn : 0u;
-                    const bool need = nn != kNoHole && hx[nn_safe] == kDeadHoleX;
-                    const uint64_t mm = ballot(need);
-                    if (need) rq[nq + lanes_below(mm)] = ((uint32_t)p << 16) | (st & 0xFFFFu);
-                    nq += __popcll(mm);
-                }
-                __syncthreads();
-                // ... and search them lane-parallel
-                for (int q0 = 0; q0 < nq; q0 += kWave) {
-                    const int i = q0 + lane;
-                    const bool act = i < nq;
-                    const uint32_t q = act ? rq[i] : 0u;
-                    const uint32_t o = q & 0xFFFFu;
-                    float x = 0.0f, y = 0.0f, z = 0.0f;
-                    if (act) egen.xyz(o, x, y, z);
-                    const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
-                    if (act) {
-                        uint32_t slot;
-                        float w;
-                        nn_from_key(key, alpha_f, slot, w);
-                        est[q >> 16] = (slot << 16) | o;
-                        ew[q >> 16] = w;
-                    }
-                }
-                __syncthreads();
                 if (round + 1 == MCL_MAX_ROUNDS) break;
+                const uint64_t lrows = rows_any(lose, rows);
+                if (lrows == 0) break;
 
-                // a loser retries on its new nearest hole in the same step, accepted
-                // with probability T_new / T_old (thinning of the draw it already won)
-                const uint64_t lmask = wave_or_u64(lose);
-                if (lmask == 0) break;
+                // a loser's hole is gone: search again now, then it retries in the same
+                // step, accepted with probability T_new / T_old (thinning of the draw
+                // it already won).  Other electrons that lost their hole are searched
+                // again at the start of the next step.
+                int nl = 0;
+                m = lrows;
+                while (m != 0) {
+                    const int r = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int p = r * kWave + lane;
+                    const bool l = (lose >> r) & 1ull;
+                    const uint64_t mm = ballot(l);
+                    if (l) rq[nl + lanes_below(mm)] = ((uint32_t)p << 16) | (est[p] & 0xFFFFu);
+                    nl += __popcll(mm);
+                }
+                __syncthreads();
+                search_queue(nl);
+                __syncthreads();
                 cand = 0;
                 const uint32_t c3r = psim | (MCL_TAG_RETRY0 + (uint32_t)round);
-                m = lmask;
+                m = lrows;
                 while (m != 0) {
                     const int r = __builtin_ctzll(m);
                     m &= m - 1;
@@ -489,12 +544,13 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
                         }
                     }
                 }
-                rowmask = wave_or_u64(cand);
+                rowmask = rows_any(cand, rows);
                 if (rowmask == 0) break;
             }
             n_live -= (int)wins;
 
-            // lazy stable compaction when a 64-slot row can be dropped
+            // lazy stable compaction when a 64-slot row can be dropped (an electron
+            // waiting for a new search moves with its stale nearest hole)
             if ((n_live + kWave - 1) / kWave < rows) {
                 int base = 0;
                 for (int r = 0; r < rows; ++r) {
@@ -529,6 +585,25 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     }
     for (int b = bin + lane; b < nb; b += kWave) cnt[b] = 0u;
     if (a.live_steps != nullptr && lane == 0) a.live_steps[c] = lsteps;
+
+    if (a.dump.enabled) {
+        // searches deferred to the next step happen now, so the dump shows every
+        // survivor's current nearest hole
+        __syncthreads();
+        int nq = 0;
+        for (int r = 0; r * kWave < n_slots; ++r) {
+            const int p = r * kWave + lane;
+            const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
+            const bool s = hole_dead(st >> 16);
+            const uint64_t m = ballot(s);
+            if (s) rq[nq + lanes_below(m)] = ((uint32_t)p << 16) | (st & 0xFFFFu);
+            nq += __popcll(m);
+        }
+        __syncthreads();
+        search_queue(nq);
+        __syncthreads();
+    }
+
     // lanes counted their own evaluations in the lane-private searches
     uint64_t evals_total = dist_evals;
 #pragma unroll
@@ -546,7 +621,6 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
 
     if (a.dump.enabled) {
         const StateDump& d = a.dump;
-        __syncthreads();
         for (int e = lane; e < n_e0; e += kWave) {
             float x, y, z;
             egen.xyz((uint32_t)e, x, y, z);
@@ -563,17 +637,18 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
             const uint32_t o = st & 0xFFFFu, slot = st >> 16;
             if (st != kDeadSlot && o < (uint32_t)n_e0) {
                 d.e_live[o] = 1;
-                d.nn[o] = slot == kNoHole ? kNoHole : (hmeta[slot] & 0xFFFFu);
+                d.nn[o] = slot == kNoHole ? kNoHole : (hole_u[4 * slot + 3] & 0xFFFFu);
                 d.w[o] = ew[p];
             }
         }
         for (int s = lane; s < n_h; s += kWave) {
-            const uint32_t h = hmeta[s] & 0xFFFFu;
+            const float4 hv = hole[s];
+            const uint32_t h = __float_as_uint(hv.w) & 0xFFFFu;
             if (h >= (uint32_t)n_h) continue;
-            d.h_alive[h] = hx[s] != kDeadHoleX;
-            d.h_xyz[3 * h] = hx[s];
-            d.h_xyz[3 * h + 1] = hy[s];
-            d.h_xyz[3 * h + 2] = hz[s];
+            d.h_alive[h] = hv.x != kDeadHoleX;
+            d.h_xyz[3 * h] = hv.x;
+            d.h_xyz[3 * h + 1] = hv.y;
+            d.h_xyz[3 * h + 2] = hv.z;
         }
     }
 }

@@ -334,6 +334,30 @@ int mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_hol
     return MCL_OK;
 }
 
+int mcl_tune_set(void* ctx, const char* key, double value)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !key) return MCL_EINVAL;
+    if (std::strcmp(key, "holes_per_cell") == 0) {
+        if (!(value >= 0.25 && value <= 64.0))
+            return fail(c, MCL_EINVAL, "holes_per_cell must be in [0.25, 64] (got %g)", value);
+        c->holes_per_cell = value;
+        return MCL_OK;
+    }
+    return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
+}
+
+int mcl_tune_get(const void* ctx, const char* key, double* value)
+{
+    const Ctx* c = (const Ctx*)ctx;
+    if (!c || !key || !value) return MCL_EINVAL;
+    if (std::strcmp(key, "holes_per_cell") == 0) {
+        *value = c->holes_per_cell;
+        return MCL_OK;
+    }
+    return MCL_EINVAL;
+}
+
 int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out)
 {
     if (!run || !out) return MCL_EINVAL;
