@@ -126,9 +126,11 @@ int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out);
  * conflicts over a shared nearest hole go to the lowest electron index, the
  * rule of src/est_params/functions.py:228-239).
  *
- *   counts     [total_clusters][n_bins] recombinations per bin; bin of step n is
- *              n * n_bins / n_steps (n_bins <= n_steps of every run; n_bins ==
- *              n_steps gives per-step counts, the reference's Lum column).
+ *   counts     [total_clusters][row_len] recombinations per bin; bin of step n is
+ *              n * n_bins / n_steps (n_bins <= n_steps of every run) and
+ *              row_len = n_bins.  n_bins = 0 gives per-step counts (the
+ *              reference's Lum column) with row_len = the longest n_steps, each
+ *              row zero-padded after its own n_steps.
  *   live_steps [total_clusters] live electron-steps per cluster, or NULL.
  * The Dose filling path (D > 0) is not part of the fixed-dt kernel: runs with
  * D != 0 are rejected with MCL_EINVAL, matching simulate.py:466 which always
@@ -142,12 +144,13 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs,
  * Plans: upload the parameter sets once, launch many times.  A plan owns the
  * device-side run descriptors and cluster map (and, with MCL_F_GRAPH, a
  * captured hipGraph).  mcl_plan_launch is asynchronous on the context's
- * stream and takes DEVICE pointers for counts [total_clusters][n_bins] and
+ * stream and takes DEVICE pointers for counts [total_clusters][row_len] and
  * live_steps [total_clusters] (may be NULL); mcl_sync waits for it.
  */
 typedef struct mcl_plan_info {
     int64_t total_clusters;
     int32_t n_bins;
+    int32_t row_len;          /* counts row stride                               */
     int32_t lds_bytes;        /* dynamic LDS per workgroup                       */
     int32_t max_electrons;    /* largest n_e0 in the plan                        */
     int32_t max_holes;        /* largest n_h in the plan                         */

@@ -80,7 +80,8 @@ class Stats(C.Structure):
 
 
 class PlanInfo(C.Structure):
-    _fields_ = [("total_clusters", C.c_int64), ("n_bins", C.c_int32), ("lds_bytes", C.c_int32),
+    _fields_ = [("total_clusters", C.c_int64), ("n_bins", C.c_int32), ("row_len", C.c_int32),
+                ("lds_bytes", C.c_int32),
                 ("max_electrons", C.c_int32), ("max_holes", C.c_int32), ("grid", C.c_int32),
                 ("block", C.c_int32)]
 
@@ -220,13 +221,15 @@ class Context:
 
     def run_fixed_dt(self, runs: Sequence[RunParams], seed: int = DEFAULT_SEED,
                      n_bins: Optional[int] = None):
-        """Run every cluster of every run; returns (counts [clusters, n_bins] uint32,
-        live_steps [clusters] uint64, Stats)."""
+        """Run every cluster of every run; returns (counts [clusters, row_len] uint32,
+        live_steps [clusters] uint64, Stats).  n_bins=None or 0: per-step counts,
+        rows as long as the longest run."""
         arr = run_array(runs)
         if n_bins is None:
-            n_bins = min(int(r.n_steps) for r in runs)
+            n_bins = 0
         total = sum(int(r.sims) for r in runs)
-        counts = np.zeros((total, n_bins), dtype=np.uint32)
+        row_len = n_bins if n_bins > 0 else max(int(r.n_steps) for r in runs if r.sims > 0)
+        counts = np.zeros((total, row_len), dtype=np.uint32)
         live = np.zeros(total, dtype=np.uint64)
         stats = Stats()
         st = self.lib.mcl_run_fixed_dt(self.handle, arr, len(runs), C.c_uint64(seed), int(n_bins),
@@ -263,7 +266,7 @@ class Plan:
         self.ctx = ctx
         arr = run_array(runs)
         if n_bins is None:
-            n_bins = min(int(r.n_steps) for r in runs)
+            n_bins = 0
         h = C.c_void_p()
         ctx.check(ctx.lib.mcl_plan_fixed_dt(ctx.handle, arr, len(runs), C.c_uint64(seed),
                                             int(n_bins), int(flags), C.byref(h)),

@@ -315,8 +315,8 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     __syncthreads();
 
     // ---- step loop (simulate.py:475-516) -------------------------------------------
-    const int nb = a.n_bins;
-    uint32_t* cnt = a.counts + (size_t)c * (size_t)nb;
+    const int nb = a.n_bins > 0 ? a.n_bins : n_steps;   // n_bins == 0: one bin per step
+    uint32_t* cnt = a.counts + (size_t)c * (size_t)a.row_len;
     const bool two_ch = R.two_channel != 0;
     const bool always2 = R.always_chan2 != 0;
     const uint64_t ret_thr = R.ret_thr;
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         if (lane == 0) cnt[bin] = bin_acc;
         ++bin;
     }
-    for (int b = bin + lane; b < nb; b += kWave) cnt[b] = 0u;
+    for (int b = bin + lane; b < a.row_len; b += kWave) cnt[b] = 0u;
     if (a.live_steps != nullptr && lane == 0) a.live_steps[c] = lsteps;
 
     if (a.dump.enabled) {

@@ -39,6 +39,7 @@ struct Plan {
     uint32_t key0 = 0, key1 = 0;
     int64_t total = 0;
     int32_t n_bins = 0;
+    int32_t row_len = 0;          // counts row stride
     int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0;
     int32_t max_e = 0, max_h = 0;
     uint32_t flags = 0;
@@ -103,8 +104,9 @@ int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geo
     const double T1 = T0 + p.T_rate * ((double)(p.n_steps - 1) * p.dt);
     if (!(T0 > 0.0) || !(T1 > 0.0))
         return fail(c, MCL_EINVAL, "run %d: temperature must stay > 0 K (%g .. %g K)", r, T0, T1);
-    if (n_bins <= 0 || n_bins > p.n_steps)
-        return fail(c, MCL_EOVERFLOW, "run %d: n_bins=%d must be in [1, n_steps=%d]", r, n_bins,
+    if (n_bins < 0) return fail(c, MCL_EINVAL, "run %d: n_bins must be >= 0", r);
+    if (n_bins > p.n_steps)
+        return fail(c, MCL_EOVERFLOW, "run %d: n_bins=%d must be in [0, n_steps=%d]", r, n_bins,
                     p.n_steps);
     geometry(p, g);
     if (g.n_e0 < 0 || g.n_h < 0) return fail(c, MCL_EINVAL, "run %d: negative particle count", r);
@@ -162,7 +164,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (!runs || n_runs <= 0) return fail(c, MCL_EINVAL, "no runs");
     std::vector<mcl::RunDesc> desc((size_t)n_runs);
     int64_t total = 0;
-    int max_e = 0, max_h = 0, max_cells = 1;
+    int max_e = 0, max_h = 0, max_cells = 1, max_steps = 0;
     for (int r = 0; r < n_runs; ++r) {
         mcl_geometry g;
         int st = validate_run(c, runs[r], r, n_bins, g);
@@ -172,6 +174,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
         if (runs[r].sims > 0) {
             if (g.n_e0 > max_e) max_e = g.n_e0;
             if (g.n_h > max_h) max_h = g.n_h;
+            if (runs[r].n_steps > max_steps) max_steps = runs[r].n_steps;
             const int G = desc[(size_t)r].grid;
             if (G * G * G > max_cells) max_cells = G * G * G;
         }
@@ -200,6 +203,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->key1 = (uint32_t)(seed >> 32);
     p->total = total;
     p->n_bins = n_bins;
+    p->row_len = n_bins > 0 ? n_bins : max_steps;
     p->h_cap = h_cap;
     p->e_cap = e_cap;
     p->g_cap = g_cap;
@@ -243,6 +247,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.key0 = p->key0;
     a.key1 = p->key1;
     a.n_bins = p->n_bins;
+    a.row_len = p->row_len;
     a.counts = counts;
     a.live_steps = live;
     a.work = p->d_work;
@@ -385,6 +390,7 @@ int mcl_plan_info_get(const void* plan, mcl_plan_info* info)
     if (!p || !info) return MCL_EINVAL;
     info->total_clusters = p->total;
     info->n_bins = p->n_bins;
+    info->row_len = p->row_len;
     info->lds_bytes = p->lds;
     info->max_electrons = p->max_e;
     info->max_holes = p->max_h;
@@ -457,7 +463,7 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
     int st = make_plan(c, runs, n_runs, nullptr, seed, n_bins, flags, &p);
     if (st) return st;
     const bool dev = (flags & MCL_F_DEVICE_PTRS) != 0;
-    const size_t n_counts = (size_t)p->total * (size_t)n_bins;
+    const size_t n_counts = (size_t)p->total * (size_t)p->row_len;
     uint32_t* d_counts = dev ? counts : nullptr;
     uint64_t* d_live = nullptr;
     hipError_t e = hipSuccess;

@@ -45,8 +45,9 @@ struct LaunchArgs {
     const int32_t* c_sim;   // [n_clusters] sim index within its run
     int32_t n_clusters;
     uint32_t key0, key1;    // Philox key = seed
-    int32_t n_bins;
-    uint32_t* counts;       // [n_clusters][n_bins]
+    int32_t n_bins;         // bins per cluster; 0 = one bin per step
+    int32_t row_len;        // counts row stride: n_bins, or the longest n_steps
+    uint32_t* counts;       // [n_clusters][row_len]
     uint64_t* live_steps;   // [n_clusters] or null
     uint64_t* work;         // [n_clusters][4] or null: live electron-steps, distance
                             // evaluations, recombinations, steps simulated

@@ -1,0 +1,132 @@
+"""The Monte Carlo driver: the drop-in for the reference's src/class/simulate.py.
+
+``simulate(cfg)`` keeps the reference's contract (simulate.py:11-112):
+
+  * runs come from ``initialize_runs(cfg)`` (zip expansion of list-valued keys);
+  * every run has ``exp_type_fp.sims`` clusters (the reference uses the first
+    run's ``sims`` and ``steps`` for all, simulate.py:24-27);
+  * it returns ``(x_ax, Lum, electron_ratio, configs)`` with arrays of shape
+    ``[steps, sims, n_runs]`` (float64), zero-padded after each run ends;
+  * it writes ``results/simulations/exp_{tag}.csv`` with columns
+    ``run,sim,step,lum,electron_ratio``, run-major then sim then step.
+
+What runs underneath is one GPU launch for every (run, sim) cluster: the
+fixed-dt step of the kernels (several recombinations per step).  So ``Lum`` is
+the number of recombinations in each step (0/1 in the reference's one-event
+steps), ``x_ax`` the end time of each step, and ``electron_ratio`` the trapped
+fraction after it.  The step is the reference's temperature-step cap
+``max_dt / T_rate`` for ramps (simulate.py:468-469) and ``dt`` (or ``max_dt``)
+for isothermal runs.  A schedule that needs more than ``steps`` steps raises
+IndexError, as the reference does when its step buffer overflows.
+
+Command line, like ``python src/class/simulate.py [overrides]``:
+
+    python -m mcluminescence_amd.simulate +tag=TLwithDose exp_type_fp.e_ratio_start=0.05
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+from pathlib import Path
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import paths
+from ._abi import DEFAULT_SEED, Context
+from .config import Config, compose
+from .engine import initialize_runs, run_params, schedule
+
+_CTX: Optional[Context] = None
+
+
+def default_context() -> Context:
+    """Process-wide libmcl context on HIP device 0 (created on first use)."""
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+def run_sweep(configs: Dict[int, Config], steps: int, sims: int, *, ctx: Optional[Context] = None,
+              seed: int = DEFAULT_SEED):
+    """One kernel launch over every run x sim; returns x_ax, Lum, electron_ratio
+    ([steps, sims, n_runs] float64) and the per-run schedules."""
+    ctx = ctx or default_context()
+    n_runs = len(configs)
+    scheds = [schedule(configs[r]) for r in range(n_runs)]
+    for r, sch in enumerate(scheds):
+        if sch["n_steps"] > steps:
+            raise IndexError(f"run {r} needs {sch['n_steps']} steps of {sch['dt']:g} s to cover "
+                             f"{sch['duration']:g} s but exp_type_fp.steps = {steps}")
+    params = [run_params(configs[r], sims=sims) for r in range(n_runs)]
+    counts, _, _ = ctx.run_fixed_dt(params, seed=seed, n_bins=0)
+    x_ax = np.zeros((steps, sims, n_runs))
+    lum = np.zeros((steps, sims, n_runs))
+    e_ratio = np.zeros((steps, sims, n_runs))
+    for r in range(n_runs):
+        p = params[r]
+        n = int(p.n_steps)
+        n0 = int(p.N_e * p.e_ratio_start)
+        c = counts[r * sims:(r + 1) * sims, :n].astype(np.float64)      # [sims, n]
+        left = n0 - np.cumsum(c, axis=1)
+        x_ax[:n, :, r] = (np.arange(1, n + 1, dtype=np.float64) * p.dt)[:, None]
+        lum[:n, :, r] = c.T
+        e_ratio[:n, :, r] = (left / float(p.N_e)).T
+    return x_ax, lum, e_ratio, scheds
+
+
+def results_frame(lum: np.ndarray, e_ratio: np.ndarray):
+    """The reference's CSV layout (simulate.py:519-532): run-major, then sim, then step."""
+    import pandas as pd
+    steps, sims, runs = lum.shape
+    return pd.DataFrame({
+        "run": np.repeat(np.arange(runs), sims * steps),
+        "sim": np.tile(np.repeat(np.arange(sims), steps), runs),
+        "step": np.tile(np.arange(steps), runs * sims),
+        "lum": lum.transpose(2, 1, 0).ravel(),
+        "electron_ratio": e_ratio.transpose(2, 1, 0).ravel(),
+    })
+
+
+def write_results_csv(path: Path, lum: np.ndarray, e_ratio: np.ndarray) -> Path:
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    results_frame(lum, e_ratio).to_csv(path, index=False)
+    return path
+
+
+def simulate(cfg: Config, *, ctx: Optional[Context] = None, seed: int = DEFAULT_SEED,
+             write_csv: bool = True, results_dir: Optional[Path] = None
+             ) -> Tuple[np.ndarray, np.ndarray, np.ndarray, Dict[int, Config]]:
+    configs = initialize_runs(cfg)
+    mc0 = configs[0].exp_type_fp
+    steps = int(mc0.steps)
+    sims = int(mc0.sims)
+    x_ax, lum, e_ratio, _ = run_sweep(configs, steps, sims, ctx=ctx, seed=seed)
+    if write_csv:
+        tag = cfg.get("tag", "")
+        out_dir = Path(results_dir) if results_dir is not None else paths.simulations_dir()
+        write_results_csv(out_dir / f"exp_{tag}.csv", lum, e_ratio)
+    return x_ax, lum, e_ratio, configs
+
+
+def main(argv: Optional[Sequence[str]] = None) -> int:
+    ap = argparse.ArgumentParser(description="Run the Monte Carlo sweep of a config and write "
+                                             "results/simulations/exp_{tag}.csv")
+    ap.add_argument("--config-dir", type=Path, default=None,
+                    help="directory holding config_fp.yaml (default: the package's conf/)")
+    ap.add_argument("--config-name", default="config_fp")
+    ap.add_argument("--seed", type=lambda s: int(s, 0), default=DEFAULT_SEED)
+    ap.add_argument("overrides", nargs="*", help="Hydra-style overrides: key=value, +key=value")
+    args = ap.parse_args(argv)
+    cfg = compose(args.overrides, config_dir=args.config_dir, config_name=args.config_name)
+    x_ax, lum, e_ratio, configs = simulate(cfg, seed=args.seed)
+    tag = cfg.get("tag", "")
+    print(f"{len(configs)} runs x {lum.shape[1]} sims: {int(lum.sum())} recombinations; wrote "
+          f"{paths.simulations_dir() / f'exp_{tag}.csv'}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -324,7 +324,8 @@ static inline uint64_t electron_thr(const cluster_t* c, const mcl_run_params* p,
 static int validate(const mcl_run_params* p, int n_bins)
 {
     if (!(p->dt > 0) || p->n_steps <= 0 || p->sims < 0) return MCL_EINVAL;
-    if (n_bins <= 0 || n_bins > p->n_steps) return MCL_EOVERFLOW;
+    if (n_bins < 0) return MCL_EINVAL;
+    if (n_bins > p->n_steps) return MCL_EOVERFLOW;
     if (p->D != 0.0) return MCL_EINVAL;
     if (!(p->k_b > 0) || !(p->T_start + 273.15 > 0)) return MCL_EINVAL;
     mcl_geometry g;
@@ -343,14 +344,16 @@ typedef struct {
     float* h_xyz;             /* [n_h][3]                                       */
 } final_state_t;
 
-/* Simulate one cluster; counts[n_bins] (zeroed here), returns live electron-steps. */
+/* Simulate one cluster; counts[row_len] (zeroed here), returns live electron-steps.
+ * n_bins == 0: one bin per step. */
 static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uint32_t key[2],
-                            int n_bins, int fp64, uint32_t* counts, final_state_t* fs)
+                            int n_bins, int row_len, int fp64, uint32_t* counts, final_state_t* fs)
 {
     cluster_t c;
     cluster_init(&c, p, run, sim, key, fp64);
-    memset(counts, 0, sizeof(uint32_t) * (size_t)n_bins);
+    memset(counts, 0, sizeof(uint32_t) * (size_t)row_len);
     const int n_steps = p->n_steps;
+    if (n_bins == 0) n_bins = n_steps;
     const double T0_K = p->T_start + 273.15;
     const double alpha_nm = p->alpha * 1e-9;
     const float alpha_f = (float)alpha_nm;
@@ -484,10 +487,12 @@ int orc_run_fixed_dt(const mcl_run_params* runs, int32_t n_runs, uint64_t seed,
 {
     if (!runs || n_runs <= 0 || !counts) return MCL_EINVAL;
     int64_t total = 0;
+    int row_len = n_bins;
     for (int r = 0; r < n_runs; ++r) {
         int st = validate(&runs[r], n_bins);
         if (st) return st;
         total += runs[r].sims;
+        if (n_bins == 0 && runs[r].sims > 0 && runs[r].n_steps > row_len) row_len = runs[r].n_steps;
     }
     int32_t* c_run = xcalloc((size_t)total, sizeof(int32_t));
     int32_t* c_sim = xcalloc((size_t)total, sizeof(int32_t));
@@ -500,8 +505,8 @@ int orc_run_fixed_dt(const mcl_run_params* runs, int32_t n_runs, uint64_t seed,
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (int64_t i = 0; i < total; ++i) {
-        uint64_t ls = run_cluster(&runs[c_run[i]], c_run[i], c_sim[i], key, n_bins, fp64,
-                                  counts + (size_t)i * (size_t)n_bins, NULL);
+        uint64_t ls = run_cluster(&runs[c_run[i]], c_run[i], c_sim[i], key, n_bins, row_len, fp64,
+                                  counts + (size_t)i * (size_t)row_len, NULL);
         if (live_steps) live_steps[i] = ls;
     }
     free(c_run); free(c_sim);
@@ -537,7 +542,7 @@ int orc_final_state(const mcl_run_params* p, int32_t run, int32_t sim, uint64_t 
     if (st) return st;
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     final_state_t fs = {max_steps, e_live, h_alive, nn, e_xyz, h_xyz};
-    run_cluster(p, run, sim, key, n_bins, 0, counts, &fs);
+    run_cluster(p, run, sim, key, n_bins, n_bins ? n_bins : p->n_steps, 0, counts, &fs);
     return 0;
 }
 

@@ -84,9 +84,10 @@ def run_fixed_dt(runs: Sequence[RunParams], seed: int = DEFAULT_SEED, n_bins=Non
     """Replay harness: counts [clusters, n_bins] uint32 and live_steps [clusters] uint64."""
     arr = run_array(runs)
     if n_bins is None:
-        n_bins = min(int(r.n_steps) for r in runs)
+        n_bins = 0
     total = sum(int(r.sims) for r in runs)
-    counts = np.zeros((total, n_bins), np.uint32)
+    row_len = n_bins if n_bins > 0 else max(int(r.n_steps) for r in runs if r.sims > 0)
+    counts = np.zeros((total, row_len), np.uint32)
     live = np.zeros(total, np.uint64)
     st = lib().orc_run_fixed_dt(arr, len(runs), C.c_uint64(seed), int(n_bins), int(bool(fp64)),
                                 int(threads), counts.ctypes.data, live.ctypes.data)
