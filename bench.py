@@ -92,8 +92,8 @@ def main() -> int:
     ctx = Context(0)
     ctx.set_stream(stream.cuda_stream)
     runs = W.iso_decay_4096x400(sims=args.sims)
-    plan = ctx.plan(runs)
-    n, nb = plan.info.total_clusters, plan.info.n_bins
+    plan = ctx.plan(runs)                     # per-step counts: n_bins = 0, rows of n_steps
+    n, nb = plan.info.total_clusters, plan.info.row_len
     counts = torch.zeros((n, nb), dtype=torch.int32, device=dev)
     live = torch.zeros(n, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()

@@ -109,6 +109,8 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
 /* Kernel tuning knobs of a context, applied to plans made afterwards.  Keys:
  *   "holes_per_cell"  mean holes per cell of each cluster's nearest-hole grid
  *                     (default 2; results do not depend on it, only speed).
+ *   "persist_frac"    persistent workgroups per size class, as a fraction of
+ *                     what fits on the device at once (default 1).
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

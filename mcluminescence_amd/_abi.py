@@ -220,7 +220,7 @@ class Context:
         return e.value, h.value
 
     def run_fixed_dt(self, runs: Sequence[RunParams], seed: int = DEFAULT_SEED,
-                     n_bins: Optional[int] = None):
+                     n_bins: Optional[int] = None, flags: int = 0):
         """Run every cluster of every run; returns (counts [clusters, row_len] uint32,
         live_steps [clusters] uint64, Stats).  n_bins=None or 0: per-step counts,
         rows as long as the longest run."""
@@ -232,8 +232,9 @@ class Context:
         counts = np.zeros((total, row_len), dtype=np.uint32)
         live = np.zeros(total, dtype=np.uint64)
         stats = Stats()
+        flags = int(flags) & ~F_DEVICE_PTRS   # host arrays here
         st = self.lib.mcl_run_fixed_dt(self.handle, arr, len(runs), C.c_uint64(seed), int(n_bins),
-                                       0, _ptr(counts), _ptr(live), C.byref(stats))
+                                       flags, _ptr(counts), _ptr(live), C.byref(stats))
         self.check(st, "mcl_run_fixed_dt")
         return counts, live, stats
 

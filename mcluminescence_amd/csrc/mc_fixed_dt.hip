@@ -214,11 +214,10 @@ __device__ __forceinline__ uint64_t rows_any(uint64_t bits, int rows)
     return m;
 }
 
-__global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
+// Simulate cluster c from its geometry to its last step, all state in lds_raw.
+__device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
+                                            unsigned char* lds_raw)
 {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const int c = blockIdx.x;
-    if (c >= a.n_clusters) return;
     const int lane = (int)threadIdx.x;
     const int run = uniform(a.c_run[c]);
     const int sim = uniform(a.c_sim[c]);
@@ -653,17 +652,60 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     }
 }
 
+// One workgroup per cluster (queue == null: cluster order[blockIdx.x], or
+// blockIdx.x when order is null), or persistent workgroups that take clusters
+// from order[] through a global atomic ticket until the list is exhausted.
+// Every workgroup leaves the loop when the ticket passes n_order.
+__global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if (a.queue == nullptr) {
+        const int i = blockIdx.x;
+        if (i < a.n_order) run_cluster(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        return;
+    }
+    for (;;) {
+        int i = 0;
+        if (threadIdx.x == 0) i = (int)atomicAdd(a.queue, 1u);
+        i = __shfl(i, 0);
+        if (i >= a.n_order) break;
+        run_cluster(a, a.order[i], lds_raw);
+        __syncthreads();
+    }
+}
+
+// Zero the tickets of the persistent class kernels (a kernel node, so the
+// graph holds no memset node).
+__global__ void reset_tickets_kernel(uint32_t* tickets, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) tickets[i] = 0u;
+}
+
 hipError_t prepare_fixed_dt(int lds_bytes)
 {
     return hipFuncSetAttribute((const void*)fixed_dt_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
 }
 
-hipError_t launch_fixed_dt(const LaunchArgs& a, int lds_bytes, hipStream_t stream)
+hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, hipStream_t stream)
 {
-    hipLaunchKernelGGL(fixed_dt_kernel, dim3((unsigned)a.n_clusters), dim3(kWave),
-                       (unsigned)lds_bytes, stream, a);
+    hipLaunchKernelGGL(fixed_dt_kernel, dim3((unsigned)grid), dim3(kWave), (unsigned)lds_bytes,
+                       stream, a);
     return hipGetLastError();
+}
+
+hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(reset_tickets_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0u, stream,
+                       tickets, n);
+    return hipGetLastError();
+}
+
+hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)fixed_dt_kernel,
+                                                        kWave, (size_t)lds_bytes);
 }
 
 }  // namespace mcl

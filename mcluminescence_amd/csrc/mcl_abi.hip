@@ -7,6 +7,7 @@
 // no HIP device is usable every entry point fails with MCL_EDEVICE.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -27,6 +28,7 @@ struct Ctx {
     int max_lds = 0;        // dynamic LDS bytes one workgroup may use
     int n_cu = 0;
     double holes_per_cell = 2.0;   // hole-grid density target (tunable)
+    double persist_frac = 1.0;     // persistent workgroups per class / device capacity
     std::string err;
 };
 
@@ -45,6 +47,22 @@ struct Plan {
     uint32_t flags = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    // persistent launch: one class kernel per cluster-size class
+    struct Class {
+        int32_t first = 0, count = 0;   // slice of d_order (clusters, most work first)
+        int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0, grid = 0;
+        int32_t max_e = 0, max_h = 0;
+    };
+    std::vector<Class> classes;
+    int32_t* d_order = nullptr;
+    uint32_t* d_tickets = nullptr;
+    std::vector<hipStream_t> side;      // fork streams for concurrent class kernels
+    std::vector<hipEvent_t> join;       // one per class
+    hipEvent_t fork = nullptr;
+    // graph of the persistent launch, instantiated for one (counts, live) pair
+    hipGraphExec_t gexec = nullptr;
+    uint32_t* g_counts = nullptr;
+    uint64_t* g_live = nullptr;
 };
 
 int fail(Ctx* c, int code, const char* fmt, ...)
@@ -152,9 +170,96 @@ void plan_free(Plan* p)
     hipFree(p->d_crun);
     hipFree(p->d_csim);
     hipFree(p->d_work);
+    hipFree(p->d_order);
+    hipFree(p->d_tickets);
+    if (p->gexec) hipGraphExecDestroy(p->gexec);
+    for (hipStream_t s : p->side) hipStreamDestroy(s);
+    for (hipEvent_t e : p->join) hipEventDestroy(e);
+    if (p->fork) hipEventDestroy(p->fork);
     if (p->ev0) hipEventDestroy(p->ev0);
     if (p->ev1) hipEventDestroy(p->ev1);
     delete p;
+}
+
+// Persistent plans: bucket the clusters by electron count (powers of two from
+// 128) so each class kernel sizes its LDS for its own clusters, order each class
+// by expected work (electrons x steps, largest first) and give it as many
+// persistent workgroups as fit on the device at once.
+int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
+                  const std::vector<mcl::RunDesc>& desc)
+{
+    struct Item { int32_t cluster; int32_t cls; double work; };
+    std::vector<Item> items;
+    items.reserve((size_t)p->total);
+    int64_t k = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        int cls = 0;
+        while ((128 << cls) < desc[(size_t)r].n_e0) ++cls;
+        const double work = (double)(desc[(size_t)r].n_e0 + 1) * (double)runs[r].n_steps;
+        for (int s = 0; s < runs[r].sims; ++s, ++k) items.push_back({(int32_t)k, cls, work});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return a.cls != b.cls ? a.cls > b.cls : a.work > b.work;
+    });
+    std::vector<int32_t> order(items.size());
+    p->classes.clear();
+    for (size_t i = 0; i < items.size(); ++i) {
+        order[i] = items[i].cluster;
+        if (i == 0 || items[i].cls != items[i - 1].cls) {
+            Plan::Class cl;
+            cl.first = (int32_t)i;
+            p->classes.push_back(cl);
+        }
+        ++p->classes.back().count;
+    }
+    // class capacities from the runs of their clusters
+    std::vector<int32_t> c_run((size_t)p->total);
+    k = 0;
+    for (int r = 0; r < n_runs; ++r)
+        for (int s = 0; s < runs[r].sims; ++s, ++k) c_run[(size_t)k] = r;
+    for (Plan::Class& cl : p->classes) {
+        int max_cells = 1;
+        for (int i = cl.first; i < cl.first + cl.count; ++i) {
+            const mcl::RunDesc& d = desc[(size_t)c_run[(size_t)order[(size_t)i]]];
+            cl.max_e = std::max(cl.max_e, d.n_e0);
+            cl.max_h = std::max(cl.max_h, d.n_h);
+            max_cells = std::max(max_cells, d.grid * d.grid * d.grid);
+        }
+        cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
+        cl.e_cap = ((cl.max_e + 63) / 64) * 64 + 64;
+        cl.g_cap = ((max_cells + 1 + 3) / 4) * 4;
+        cl.lds = mcl::fixed_dt_lds_bytes(cl.h_cap, cl.e_cap, cl.g_cap);
+        if (cl.lds > c->max_lds)
+            return fail(c, MCL_ERANGE, "cluster of %d electrons / %d holes needs %d B of LDS; a "
+                        "workgroup has %d B", cl.max_e, cl.max_h, cl.lds, c->max_lds);
+        // (the kernel's dynamic-LDS limit was raised to the plan's largest class)
+        int per_cu = 0;
+        const hipError_t e = mcl::fixed_dt_occupancy(cl.lds, &per_cu);
+        if (e != hipSuccess) return fail(c, MCL_EDEVICE, "occupancy query: %s", hipGetErrorString(e));
+        const double frac = c->persist_frac;
+        int grid = (int)std::max(1.0, std::floor((double)std::max(per_cu, 1) * c->n_cu * frac));
+        cl.grid = std::min(grid, cl.count);
+    }
+    const size_t ncls = p->classes.size();
+    hipError_t e = hipMalloc((void**)&p->d_order, sizeof(int32_t) * order.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_tickets, sizeof(uint32_t) * std::max<size_t>(ncls, 1));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+    for (size_t i = 0; i < ncls && e == hipSuccess; ++i) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) p->join.push_back(ev);
+        if (e == hipSuccess && i > 0) {
+            hipStream_t s = nullptr;
+            e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);   // default priority
+            if (e == hipSuccess) p->side.push_back(s);
+        }
+    }
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "persistent setup: %s",
+                    hipGetErrorString(e));
+    return MCL_OK;
 }
 
 // Build a plan; run_ids overrides the Philox run index of each run (may be null).
@@ -233,6 +338,13 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "plan setup: %s",
                     hipGetErrorString(e));
     }
+    if (flags & (MCL_F_PERSISTENT | MCL_F_GRAPH)) {
+        const int st = build_classes(c, p, runs, n_runs, desc);
+        if (st) {
+            plan_free(p);
+            return st;
+        }
+    }
     *out = p;
     return MCL_OK;
 }
@@ -244,6 +356,9 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.c_run = p->d_crun;
     a.c_sim = p->d_csim;
     a.n_clusters = (int32_t)p->total;
+    a.order = nullptr;
+    a.n_order = (int32_t)p->total;
+    a.queue = nullptr;
     a.key0 = p->key0;
     a.key1 = p->key1;
     a.n_bins = p->n_bins;
@@ -258,14 +373,73 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     return a;
 }
 
+// The persistent launch on stream s: reset the tickets, fork one stream per
+// class, run the class kernels concurrently, join.
+hipError_t enqueue_persistent(Plan* p, uint32_t* counts, uint64_t* live, hipStream_t s)
+{
+    const int ncls = (int)p->classes.size();
+    hipError_t e = mcl::launch_reset_tickets(p->d_tickets, ncls, s);
+    if (e == hipSuccess) e = hipEventRecord(p->fork, s);
+    for (int k = 0; k < ncls && e == hipSuccess; ++k) {
+        const Plan::Class& cl = p->classes[(size_t)k];
+        hipStream_t sk = k == 0 ? s : p->side[(size_t)k - 1];
+        if (k > 0) e = hipStreamWaitEvent(sk, p->fork, 0);
+        mcl::LaunchArgs a = launch_args(p, counts, live);
+        a.order = p->d_order + cl.first;
+        a.n_order = cl.count;
+        a.queue = p->d_tickets + k;
+        a.h_cap = cl.h_cap;
+        a.e_cap = cl.e_cap;
+        a.g_cap = cl.g_cap;
+        if (e == hipSuccess) e = mcl::launch_fixed_dt(a, cl.grid, cl.lds, sk);
+        if (e == hipSuccess && k > 0) e = hipEventRecord(p->join[(size_t)k], sk);
+        if (e == hipSuccess && k > 0) e = hipStreamWaitEvent(s, p->join[(size_t)k], 0);
+    }
+    return e;
+}
+
 int plan_launch(Plan* p, uint32_t* counts, uint64_t* live, const mcl::StateDump* dump)
 {
     Ctx* c = p->ctx;
     if (!counts) return fail(c, MCL_EINVAL, "counts is null");
+    if (p->flags & MCL_F_GRAPH && !dump) {
+        if (!p->gexec || p->g_counts != counts || p->g_live != live) {
+            if (p->gexec) {
+                HIP_TRY(c, hipGraphExecDestroy(p->gexec));
+                p->gexec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_persistent(p, counts, live, c->stream);
+            hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (graph) hipGraphDestroy(graph);
+                return fail(c, MCL_EDEVICE, "graph capture: %s",
+                            hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            e = hipGraphInstantiate(&p->gexec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(c, MCL_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+            p->g_counts = counts;
+            p->g_live = live;
+        }
+        HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
+        HIP_TRY(c, hipGraphLaunch(p->gexec, c->stream));
+        HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
+        p->timed = true;
+        return MCL_OK;
+    }
+    if (p->flags & MCL_F_PERSISTENT && !dump) {
+        HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
+        HIP_TRY(c, enqueue_persistent(p, counts, live, c->stream));
+        HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
+        p->timed = true;
+        return MCL_OK;
+    }
     mcl::LaunchArgs a = launch_args(p, counts, live);
     if (dump) a.dump = *dump;
     HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
-    HIP_TRY(c, mcl::launch_fixed_dt(a, p->lds, c->stream));
+    HIP_TRY(c, mcl::launch_fixed_dt(a, (int)p->total, p->lds, c->stream));
     HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
     p->timed = true;
     return MCL_OK;
@@ -349,6 +523,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->holes_per_cell = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "persist_frac") == 0) {
+        if (!(value > 0.0 && value <= 4.0))
+            return fail(c, MCL_EINVAL, "persist_frac must be in (0, 4] (got %g)", value);
+        c->persist_frac = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -358,6 +538,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     if (!c || !key || !value) return MCL_EINVAL;
     if (std::strcmp(key, "holes_per_cell") == 0) {
         *value = c->holes_per_cell;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "persist_frac") == 0) {
+        *value = c->persist_frac;
         return MCL_OK;
     }
     return MCL_EINVAL;

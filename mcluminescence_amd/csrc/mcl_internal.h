@@ -44,6 +44,9 @@ struct LaunchArgs {
     const int32_t* c_run;   // [n_clusters] run index of each cluster
     const int32_t* c_sim;   // [n_clusters] sim index within its run
     int32_t n_clusters;
+    const int32_t* order;   // clusters this launch simulates (null: 0 .. n_order-1)
+    int32_t n_order;
+    uint32_t* queue;        // persistent launch: atomic ticket into order (null: one per block)
     uint32_t key0, key1;    // Philox key = seed
     int32_t n_bins;         // bins per cluster; 0 = one bin per step
     int32_t row_len;        // counts row stride: n_bins, or the longest n_steps
@@ -74,7 +77,9 @@ inline int grid_dim(int n_h, double holes_per_cell)
     return G;
 }
 
-hipError_t launch_fixed_dt(const LaunchArgs& a, int lds_bytes, hipStream_t stream);
+hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, hipStream_t stream);
+hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream);
 hipError_t prepare_fixed_dt(int lds_bytes);
+hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu);
 
 }  // namespace mcl

@@ -1,0 +1,73 @@
+"""Batched uneven sweeps: the persistent size-class launch and its hipGraph replay give
+exactly the counts of the one-workgroup-per-cluster launch and of the replay harness."""
+import numpy as np
+import pytest
+
+from mcluminescence_amd import autotune
+from mcluminescence_amd import workloads as W
+from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _sweep():
+    return W.uneven_sweep(n_sets=240, n_steps=192, seed=11)
+
+
+def test_persistent_equals_plain_and_oracle(gpu_ctx):
+    runs = _sweep()
+    print("uneven sweep: 240 sets, 64-2048 electrons, 192 steps, 48 bins", flush=True)
+    plain, lp, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48)
+    pers, lq, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
+    np.testing.assert_array_equal(plain, pers)
+    np.testing.assert_array_equal(lp, lq)
+    ref, lr = orc.run_fixed_dt(runs, n_bins=48, threads=8)
+    np.testing.assert_array_equal(plain, ref)
+    np.testing.assert_array_equal(lp, lr)
+
+
+def test_graph_replays_match_eager(gpu_ctx):
+    import torch
+    runs = _sweep()
+    eager, _, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48)
+    plan = gpu_ctx.plan(runs, n_bins=48, flags=F_PERSISTENT | F_GRAPH)
+    n, row = plan.info.total_clusters, plan.info.row_len
+    counts = torch.zeros((n, row), dtype=torch.int32, device="cuda")
+    live = torch.zeros(n, dtype=torch.int64, device="cuda")
+    for rep in range(4):
+        counts.fill_(-1)                       # eager work between replays
+        torch.cuda.synchronize()
+        plan.launch(counts.data_ptr(), live.data_ptr())
+        plan.sync()
+        got = counts.cpu().numpy().view(np.uint32)
+        np.testing.assert_array_equal(got, eager, err_msg=f"graph replay {rep}")
+        assert plan.work().recombinations == int(eager.sum())
+    plan.close()
+
+
+def test_tuning_knobs_do_not_change_results(gpu_ctx):
+    runs = _sweep()[:80]
+    base, _, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
+    try:
+        for hpc, pf in [(1.0, 0.5), (4.0, 2.0)]:
+            gpu_ctx.tune_set("holes_per_cell", hpc)
+            gpu_ctx.tune_set("persist_frac", pf)
+            got, _, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
+            np.testing.assert_array_equal(got, base)
+    finally:
+        gpu_ctx.tune_set("holes_per_cell", 2.0)
+        gpu_ctx.tune_set("persist_frac", 1.0)
+
+
+def test_autotuner_picks_a_measured_point(gpu_ctx, tmp_path):
+    runs = _sweep()[:60]
+    entry = autotune.autotune(gpu_ctx, runs, n_bins=48,
+                              space={"holes_per_cell": [2.0, 3.0], "persist_frac": [1.0]},
+                              reps=1, table_path=tmp_path / "t.json", log=None)
+    assert entry["holes_per_cell"] in (2.0, 3.0) and entry["ms"] > 0
+    table = autotune.load_table(tmp_path / "t.json")
+    assert autotune.signature(runs) in table
+    knobs = autotune.apply(gpu_ctx, runs, table)
+    assert knobs["holes_per_cell"] == entry["holes_per_cell"]
+    gpu_ctx.tune_set("holes_per_cell", 2.0)
