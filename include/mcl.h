@@ -111,6 +111,9 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     (default 2; results do not depend on it, only speed).
  *   "persist_frac"    persistent workgroups per size class, as a fraction of
  *                     what fits on the device at once (default 1).
+ *   "wave_search_max" electrons needing a new nearest hole in one step are
+ *                     searched one at a time by the whole wave up to this
+ *                     many, lane-parallel beyond (default 2, at most 64).
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

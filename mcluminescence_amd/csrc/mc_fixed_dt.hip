@@ -163,6 +163,56 @@ __device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, fl
     return key;
 }
 
+// Wave-cooperative search for one electron (x, y, z wave-uniform): the lanes
+// spread over the concatenated slot ranges of the 3x3x3 block, one hole each,
+// then a wave min.  About 150 VALU instructions per electron against ~1000 for
+// a lane-private pass, so it wins while few electrons are queued.
+__device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, float y, float z,
+                                                      int lane, bool& conclusive, uint64_t& evals)
+{
+    const int G = g.G;
+    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
+              cz = cell_of(z, g.inv_cell, G);
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
+    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
+    const int ny = y1 - y0 + 1;
+    const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
+    int s = 0, len = 0;
+    if (lane < nr) {
+        const int zz = z0 + lane / ny, yy = y0 + lane % ny;
+        const int base = (zz * G + yy) * G;
+        s = (int)g.cstart[base + x0];
+        len = (int)g.cstart[base + x1 + 1] - s;
+    }
+    int incl = len;                                  // inclusive prefix over lanes 0..15
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 15);
+    const int excl = incl - len;
+    if (lane == 0) evals += (uint64_t)total;         // summed over lanes at the end
+    uint64_t key = ~0ull;
+    for (int l0 = 0; l0 < total; l0 += kWave) {
+        const int l = l0 + lane;
+        int sel = 0;
+        for (int j = 1; j < nr; ++j) sel = (l >= __shfl(excl, j)) ? j : sel;
+        const int slot = __shfl(s, sel) + (l - __shfl(excl, sel));
+        if (l < total) {
+            const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
+            key = k < key ? k : key;
+        }
+    }
+    key = wave_min_u64(key);
+    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
+    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
+                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
+    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
+    return key;
+}
+
 // The whole wave scans every hole for one electron (x, y, z wave-uniform).
 __device__ __forceinline__ uint64_t full_scan_wave(const Grid& g, float x, float y, float z,
                                                    int lane, uint64_t& evals)
@@ -355,8 +405,29 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             cand |= 1ull << rr;
         }
     };
-    // Search again for the queued electrons rq[0..nq), lane-parallel.
+    // Search again for the queued electrons rq[0..nq): one electron at a time
+    // with the whole wave while the queue is short, lane-parallel otherwise.
     auto search_queue = [&](int nq) {
+        if (nq <= a.wave_search_max) {
+            float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+            const uint32_t qq = lane < nq ? rq[lane] : 0u;
+            if (lane < nq) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
+            for (int i = 0; i < nq; ++i) {
+                const uint32_t q = (uint32_t)__shfl((int)qq, i);
+                const float x = __shfl(qx, i), y = __shfl(qy, i), z = __shfl(qz, i);
+                bool ok = true;
+                uint64_t key = block_search_wave(grid, x, y, z, lane, ok, dist_evals);
+                if (!ok) key = full_scan_wave(grid, x, y, z, lane, dist_evals);
+                uint32_t slot;
+                float w;
+                nn_from_key(key, alpha_f, slot, w);
+                if (lane == 0) {
+                    est[q >> 16] = (slot << 16) | (q & 0xFFFFu);
+                    ew[q >> 16] = w;
+                }
+            }
+            return;
+        }
         for (int q0 = 0; q0 < nq; q0 += kWave) {
             const int i = q0 + lane;
             const bool act = i < nq;

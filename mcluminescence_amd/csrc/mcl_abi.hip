@@ -29,6 +29,7 @@ struct Ctx {
     int n_cu = 0;
     double holes_per_cell = 2.0;   // hole-grid density target (tunable)
     double persist_frac = 1.0;     // persistent workgroups per class / device capacity
+    double wave_search_max = 2.0;  // queue length searched one electron per wave
     std::string err;
 };
 
@@ -369,6 +370,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.h_cap = p->h_cap;
     a.e_cap = p->e_cap;
     a.g_cap = p->g_cap;
+    a.wave_search_max = (int32_t)p->ctx->wave_search_max;
     a.dump.enabled = 0;
     return a;
 }
@@ -523,6 +525,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->holes_per_cell = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "wave_search_max") == 0) {
+        if (!(value >= 0.0 && value <= 64.0))
+            return fail(c, MCL_EINVAL, "wave_search_max must be in [0, 64] (got %g)", value);
+        c->wave_search_max = std::floor(value);
+        return MCL_OK;
+    }
     if (std::strcmp(key, "persist_frac") == 0) {
         if (!(value > 0.0 && value <= 4.0))
             return fail(c, MCL_EINVAL, "persist_frac must be in (0, 4] (got %g)", value);
@@ -542,6 +550,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "persist_frac") == 0) {
         *value = c->persist_frac;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "wave_search_max") == 0) {
+        *value = c->wave_search_max;
         return MCL_OK;
     }
     return MCL_EINVAL;
