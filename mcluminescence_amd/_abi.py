@@ -1,4 +1,5 @@
-"""ctypes binding of libmcl.so, the C ABI declared in include/mcl.h.
+"""ctypes binding of libmcl.so, the C ABI declared in include/mcl.h and
+include/mcl_surrogate.h.
 
 This is the only way the package reaches the GPU.  There is no CPU fallback:
 if the library is missing or no HIP device is present, ``Library()`` and
@@ -92,6 +93,18 @@ class Work(C.Structure):
                 ("kernel_ms", C.c_double)]
 
 
+class MlpDesc(C.Structure):
+    """mcl_mlp_desc (include/mcl_surrogate.h)."""
+    _fields_ = [("n_in", C.c_int32), ("n_hidden", C.c_int32), ("n_hidden_layers", C.c_int32),
+                ("n_out", C.c_int32), ("max_batch", C.c_int32)]
+
+
+class AdamW(C.Structure):
+    """mcl_adamw (include/mcl_surrogate.h)."""
+    _fields_ = [("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("weight_decay", C.c_double), ("step", C.c_int64)]
+
+
 class MclError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mcl error {code}: {msg}")
@@ -121,6 +134,16 @@ _SIGNATURES = {
     "mcl_sync": (C.c_int, [_P]),
     "mcl_cluster_state": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
+    # surrogate network (include/mcl_surrogate.h)
+    "mcl_mlp_create": (C.c_int, [_P, C.POINTER(MlpDesc), C.POINTER(_P)]),
+    "mcl_mlp_destroy": (None, [_P]),
+    "mcl_mlp_param_count": (C.c_int64, [_P]),
+    "mcl_mlp_get_params": (C.c_int, [_P, _P]),
+    "mcl_mlp_set_params": (C.c_int, [_P, _P]),
+    "mcl_mlp_forward": (C.c_int, [_P, _P, C.c_int32, _P]),
+    "mcl_mlp_grad": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
+    "mcl_mlp_train_step": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.POINTER(AdamW), _P]),
+    "mcl_mlp_last_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

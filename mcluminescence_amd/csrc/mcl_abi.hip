@@ -17,21 +17,14 @@
 #include <vector>
 
 #include "../../include/mcl.h"
+#include "mcl_ctx.h"
 #include "mcl_internal.h"
 
 namespace {
 
-struct Ctx {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    int max_lds = 0;        // dynamic LDS bytes one workgroup may use
-    int n_cu = 0;
-    double holes_per_cell = 2.0;   // hole-grid density target (tunable)
-    double persist_frac = 1.0;     // persistent workgroups per class / device capacity
-    double wave_search_max = 2.0;  // queue length searched one electron per wave
-    std::string err;
-};
+using mcl::Ctx;
+using mcl::fail;
+
 
 struct Plan {
     Ctx* ctx = nullptr;
@@ -65,25 +58,6 @@ struct Plan {
     uint32_t* g_counts = nullptr;
     uint64_t* g_live = nullptr;
 };
-
-int fail(Ctx* c, int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    if (c) c->err = buf;
-    return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                                    \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail((ctx), e_ == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "%s: %s", \
-                        #expr, hipGetErrorString(e_));                                        \
-    } while (0)
 
 constexpr double kPi = 0x1.921fb54442d18p+1;
 

@@ -1,0 +1,92 @@
+// mlp_internal.h — descriptors shared by the surrogate kernels (mlp.hip) and
+// their host side (mlp_abi.hip); not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcl {
+namespace mlp {
+
+// What a GEMM's epilogue does with the FP32 accumulator acc[m][n] of
+// C = A B^T (A [M][K], B [N][K], both K-contiguous BF16).
+enum Epi : int {
+    kHiddenTrain = 0,  // z = acc + bias: out_z = z, out_a = silu(z), out_t = silu(z)^T
+    kHiddenInfer = 1,  // out_a = silu(acc + bias)
+    kOutInfer = 2,     // out_f = acc + bias (FP32)
+    kOutTrain = 3,     // y = acc + bias; d = y - target: out_a = d * grad_scale, out_t = its
+                       // transpose, loss_part[block] = sum d^2 (out_f = y if given)
+    kDxSilu = 4,       // dz = acc * silu'(z_in): out_a = dz (if given), out_t = dz^T
+    kDwPartial = 5,    // out_f[split][M][N] = acc over the split's K range
+    kEpiCount = 6
+};
+
+// Tile shapes (rows x columns of C per 256-thread workgroup).
+enum Tile : int { kTile128x128 = 0, kTile128x64 = 1, kTile64x64 = 2, kTileCount = 3 };
+inline int tile_m(int t) { return t == kTile64x64 ? 64 : 128; }
+inline int tile_n(int t) { return t == kTile128x128 ? 128 : 64; }
+
+struct GemmArgs {
+    const __bf16* A;          // [M][lda], this split's K range starts at z * K
+    const __bf16* B;          // [N][ldb]
+    int64_t lda, ldb;
+    int M, N, K;              // K = contraction length of one split (multiple of 32)
+    const float* bias;        // [N]
+    __bf16* out_a;            // [M][N]
+    __bf16* out_z;            // [M][N]
+    __bf16* out_t;            // [N][ldt]
+    int64_t ldt;
+    float* out_f;             // [M][N], or [splits][M][N] partial sums
+    const __bf16* z_in;       // [M][N]
+    const float* target;      // [rows][ld_target]
+    const int32_t* t_idx;     // target row of output row m (null: m)
+    int64_t ld_target;
+    float grad_scale;
+    float* loss_part;         // [workgroups]
+};
+
+hipError_t gemm(const GemmArgs& g, int tile, int epi, int splits, hipStream_t stream);
+
+// x [rows][n_in] FP32 (row idx[b], or b) -> X [batch][kp] BF16 zero-padded and,
+// if xt is given, X^T [kp][batch].
+hipError_t pack_input(const float* x, const int32_t* idx, int n_in, int batch, int kp,
+                      __bf16* X, __bf16* xt, hipStream_t stream);
+
+// loss = sum(part[0..n)) * scale, summed in a fixed order.
+hipError_t finish_loss(const float* part, int n, float scale, float* loss, hipStream_t stream);
+
+constexpr int kMaxLayers = 8;
+
+struct OptLayer {
+    float* w; float* m; float* v;     // FP32 master weight and moments [out][kp]
+    __bf16* wb;                       // BF16 copy [out][kp]
+    __bf16* wt;                       // BF16 transposed copy [kp][out]
+    const float* part;                // weight-gradient partial sums [splits][out][kp]
+    int splits;
+    float* bw; float* bm; float* bv;  // bias master and moments [out]
+    const __bf16* dzt;                // dL/dz^T [out][batch] (bias gradient = row sums)
+    int out, kp, k_in;                // k_in: unpadded inputs (flat layout)
+    int tile0, bias0;                 // first weight-tile / bias workgroup of the layer
+    int64_t flat_w, flat_b;           // offsets of W and b in the flat layout
+};
+
+enum OptMode : int { kOptAdamW = 0, kOptExportGrad = 1, kOptRefresh = 2 };
+
+struct OptArgs {
+    OptLayer L[kMaxLayers];
+    int n_layers;
+    int mode;
+    int batch;
+    float* flat;                      // kOptExportGrad destination
+    float lr, beta1, beta2, eps, wd;
+    float step_size;                  // lr / (1 - beta1^t)
+    float sqrt_bc2;                   // sqrt(1 - beta2^t)
+};
+
+// Workgroups of the optimizer kernel: one per 32x32 weight tile, one per four biases.
+__host__ __device__ inline int opt_tiles(int out, int kp) { return (out / 32) * (kp / 32); }
+__host__ __device__ inline int opt_bias_groups(int out) { return out / 4; }
+hipError_t optimizer(const OptArgs& a, int n_groups, hipStream_t stream);
+
+}  // namespace mlp
+}  // namespace mcl
