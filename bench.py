@@ -64,6 +64,92 @@ def cpu_baseline(seconds: float) -> dict:
             "curves_per_s": done / el}
 
 
+PEAK_BF16_TFLOPS = 2500.0       # dense BF16 matrix-core peak
+
+
+def tl_extras(ctx, stream) -> dict:
+    """Glow-curve throughput: the 16 x 500 TL glow workload (300 -> 700 K at 1 K/s,
+    dt 0.1 s, 4000 steps) and batched 256-bin curves of the surrogate's window
+    (1024 parameter sets x 8 clusters x 400 electrons, 2048 steps)."""
+    import numpy as np
+    import torch
+    from mcluminescence_amd import surrogate as S
+    from mcluminescence_amd import workloads as W
+    plan = ctx.plan(W.tl_glow_16x500(), n_bins=4000)
+    counts = torch.zeros((16, 4000), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    plan.launch(counts.data_ptr())
+    plan.sync()
+    ms = []
+    for _ in range(5):
+        plan.launch(counts.data_ptr())
+        plan.sync()
+        ms.append(plan.last_ms())
+    plan.close()
+    theta = S.sample_params(1024, np.random.default_rng(3))
+    S.mc_curves(ctx, theta[:64], n=400, sims=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S.mc_curves(ctx, theta, n=400, sims=8)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"tl_glow_16x500_ms": float(np.median(ms)),
+            "tl_curves_per_s": 1024 / dt,
+            "tl_curves_note": "256-bin glow curves of 8 clusters x 400 electrons each, 2048 steps, "
+                              "1024 parameter sets per call, host round trip included"}
+
+
+def surrogate_extras(ctx) -> dict:
+    """Surrogate network: one AdamW training step at batch 4096 and one inference
+    batch of 65,536 (device time, HIP events), and one surrogate-assisted fit of a
+    synthetic lab curve against the plain Monte Carlo fit of the same curve."""
+    import numpy as np
+    import torch
+    from mcluminescence_amd import fit as F
+    from mcluminescence_amd import fit_bench
+    from mcluminescence_amd import surrogate as S
+    from mcluminescence_amd._abi import AdamW
+    net = S.Network.load(ctx, fit_bench.MODEL, max_batch=65536)
+    flop = 2 * (6 * 512 + 3 * 512 * 512 + 512 * 256)      # per sample, unpadded
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = (torch.rand((65536, 6), generator=g) * 2 - 1).cuda()
+    t = torch.rand((65536, 256), generator=g).cuda()
+    for _ in range(3):
+        net.forward(x)
+    inf = []
+    for _ in range(10):
+        net.forward(x)
+        inf.append(net.last_ms())
+    idx = torch.arange(4096, dtype=torch.int32, device="cuda")
+    params = net.get_params()
+    hp = AdamW(1e-6, 0.9, 0.999, 1e-8, 0.0, 1)
+    tr = []
+    for k in range(20):
+        hp.step = k + 1
+        net.train_step(x, t, idx, hp)
+        tr.append(net.last_ms())
+    net.set_params(params)
+    inf_ms, tr_ms = float(np.median(inf)), float(np.median(tr[3:]))
+    rng = np.random.default_rng(7)
+    truth = fit_bench.draw_truth(rng)
+    y, scale = F.lab_curve(ctx, truth, rng=rng)
+    rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=150, seed=1)
+    rm = F.fit_mc(ctx, y, scale, truth[5], pop=128, gens=40, seed=1)
+    net.close()
+    return {"inference_b65536_ms": inf_ms,
+            "inference_tflops": 65536 * flop / (inf_ms * 1e-3) / 1e12,
+            "inference_frac_bf16_peak": 65536 * flop / (inf_ms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+            "train_step_b4096_ms": tr_ms,
+            "train_tflops": 3 * 4096 * flop / (tr_ms * 1e-3) / 1e12,
+            "fit_seconds_surrogate": rs.seconds, "fit_seconds_mc": rm.seconds,
+            "fit_speedup": rm.seconds / rs.seconds, "fit_judge_nrmse": rs.judge_nrmse,
+            "fit_fallback": rs.fallback,
+            "fit_objective_surrogate": F.judge_objective(ctx, rs.theta, y, scale),
+            "fit_objective_mc": F.judge_objective(ctx, rm.theta, y, scale),
+            "fit_note": "surrogate DE 1024 x 150 generations + one 64 x 1000 Monte Carlo judge "
+                        "vs plain Monte Carlo DE 128 x 40 generations (64 x 1000 per candidate)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +158,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sims", type=int, default=4096)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the glow-curve and surrogate figures")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,12 +239,16 @@ def main() -> int:
                      "hbm_gbps_algorithmic": out_bytes / (mean_ms * 1e-3) / 1e9,
                      "hbm_peak_gbps": PEAK_HBM_GBPS},
     }
+    plan.close()
+    if not args.no_extras:
+        ctx.set_stream(None)
+        result["tl"] = tl_extras(ctx, stream)
+        result["surrogate"] = surrogate_extras(ctx)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         result["cpu_baseline"]["host_cpu"] = _cpu_model()
         result["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
     print(json.dumps(result), flush=True)
-    plan.close()
     ctx.close()
     return 0
 

@@ -1,0 +1,205 @@
+"""Trap-parameter estimation from a glow curve, in the form of the reference's
+optimizer: differential evolution ("randtobest1bin", mutation 0.5, recombination
+0.3 — src/class/optimizer.py:104-110) minimising the mean squared difference
+between a simulated and a measured curve (optimizer.py:66-84 objective).
+
+Two ways to evaluate a candidate population:
+
+  fit_mc         every candidate of every generation is simulated by the Monte
+                 Carlo (one batched launch per generation) — the plain method;
+  fit_surrogate  candidates are scored by the trained network (one inference batch
+                 per generation); the best one is re-simulated by the Monte Carlo
+                 judge (64 clusters x 1000 electrons), and if the network's curve
+                 misses the judge's by more than the gate (2 % NRMSE) the search
+                 falls back to Monte Carlo generations seeded with the network's
+                 final population.
+
+The unknowns are E, s, alpha, rho and the fill fraction; the heating rate is the
+experiment's.  The curve depends on alpha and rho only through
+rho' = rho / (3/(4 pi) alpha^3) (tl_trap_lab.py:33), so recovery is reported for
+E, s, rho' and fill.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import surrogate as S
+from ._abi import DEFAULT_SEED, Context
+from .workloads import rho_prime_for
+
+N_FIT = 5          # E, s, alpha, rho, fill
+JUDGE_SIMS, JUDGE_N = 64, 1000
+
+
+def full_theta(u5: np.ndarray, beta: float) -> np.ndarray:
+    """Encoded fit vector(s) [.., 5] -> physical parameters [.., 6] with the fixed beta."""
+    u5 = np.atleast_2d(u5)
+    th = S.decode(np.concatenate([u5, np.zeros((u5.shape[0], 1))], axis=1))
+    th[:, 5] = beta
+    return th
+
+
+def lab_curve(ctx: Context, theta: np.ndarray, peak_counts: float = 1e4,
+              rng: Optional[np.random.Generator] = None, seed: int = DEFAULT_SEED + 101):
+    """Synthetic measured curve: Monte Carlo at theta (64 clusters x 1000 electrons)
+    scaled to a peak of peak_counts, with Poisson counting noise.  Returns (counts,
+    scale) where scale converts a fraction-of-N_e curve into counts."""
+    rng = rng or np.random.default_rng(0)
+    c = S.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)[0]
+    scale = peak_counts / max(c.max(), 1e-30)
+    return rng.poisson(c * scale).astype(np.float64), scale
+
+
+def objective(curves: np.ndarray, y: np.ndarray, scale: float) -> np.ndarray:
+    """Mean squared difference of scaled candidate curves and the measured curve,
+    divided by the measured peak squared (so sqrt(objective) is an NRMSE)."""
+    d = np.atleast_2d(curves) * scale - y
+    return np.mean(d * d, axis=1) / max(y.max(), 1.0) ** 2
+
+
+class DifferentialEvolution:
+    """Vectorised DE/rand-to-best/1/bin over the box [-1, 1]^d (scipy's
+    'randtobest1bin': mutant = x_r0 + F (best - x_r0) + F (x_r1 - x_r2), binomial
+    crossover with at least one mutant coordinate; coordinates leaving the box are
+    redrawn uniformly)."""
+
+    def __init__(self, d: int, pop: int, rng: np.random.Generator, F: float = 0.5,
+                 CR: float = 0.3, init: Optional[np.ndarray] = None):
+        self.d, self.P, self.F, self.CR, self.rng = d, pop, F, CR, rng
+        self.x = init.copy() if init is not None else rng.uniform(-1, 1, (pop, d))
+        self.f = np.full(pop, np.inf)
+
+    def seed_fitness(self, f: np.ndarray) -> None:
+        self.f = np.asarray(f, np.float64).copy()
+
+    def propose(self) -> np.ndarray:
+        P, d, rng = self.P, self.d, self.rng
+        best = self.x[np.argmin(self.f)]
+        # three distinct partners per target, all different from the target
+        me = np.arange(P)[:, None]
+        r = rng.integers(0, P, (P, 3))
+        for _ in range(64):
+            bad = ((r == me).any(axis=1) | (r[:, 0] == r[:, 1]) | (r[:, 0] == r[:, 2]) |
+                   (r[:, 1] == r[:, 2]))
+            if not bad.any():
+                break
+            r[bad] = rng.integers(0, P, (int(bad.sum()), 3))
+        x0, x1, x2 = self.x[r[:, 0]], self.x[r[:, 1]], self.x[r[:, 2]]
+        mutant = x0 + self.F * (best - x0) + self.F * (x1 - x2)
+        cross = rng.uniform(size=(P, d)) < self.CR
+        cross[np.arange(P), rng.integers(0, d, P)] = True
+        trial = np.where(cross, mutant, self.x)
+        bad = (trial < -1) | (trial > 1)
+        trial[bad] = rng.uniform(-1, 1, bad.sum())
+        return trial
+
+    def accept(self, trial: np.ndarray, f: np.ndarray) -> None:
+        better = f <= self.f
+        self.x[better] = trial[better]
+        self.f[better] = f[better]
+
+    @property
+    def best(self):
+        i = int(np.argmin(self.f))
+        return self.x[i].copy(), float(self.f[i])
+
+
+@dataclass
+class FitResult:
+    theta: np.ndarray                 # physical parameters [6] (beta included)
+    objective_mc: float               # objective of theta on the Monte Carlo judge curve
+    judge_nrmse: float                # network vs judge curve at theta (nan for fit_mc)
+    seconds: float
+    mc_candidates: int                # candidate curves simulated by the Monte Carlo
+    net_candidates: int               # candidate curves evaluated by the network
+    fallback: bool = False
+    history: List[float] = field(default_factory=list)
+
+    def recovery(self, truth: np.ndarray) -> dict:
+        """Errors against the true parameters: |dE| [eV], |d log10 s|, |d log10 rho'|, |d fill|."""
+        t, h = np.asarray(truth, float), self.theta
+        rp = lambda th: rho_prime_for(th[3], th[2])
+        return {"E": abs(h[0] - t[0]), "log10_s": abs(math.log10(h[1] / t[1])),
+                "log10_rho_prime": abs(math.log10(rp(h) / rp(t))), "fill": abs(h[4] - t[4])}
+
+
+def judge_objective(ctx: Context, theta: np.ndarray, y: np.ndarray, scale: float,
+                    seed: int = DEFAULT_SEED + 999) -> float:
+    """Objective of one parameter set on a fresh Monte Carlo judge curve (64 clusters x
+    1000 electrons, a seed no fit uses): the common yardstick for comparing fits."""
+    c = S.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)
+    return float(objective(c, y, scale)[0])
+
+
+def _mc_eval(ctx: Context, th: np.ndarray, sims: int, n: int, seed: int) -> np.ndarray:
+    return S.mc_curves(ctx, th, n=n, sims=sims, seed=seed)
+
+
+def fit_mc(ctx: Context, y: np.ndarray, scale: float, beta: float, pop: int = 64,
+           gens: int = 30, sims: int = JUDGE_SIMS, n: int = JUDGE_N, seed: int = 0,
+           init: Optional[np.ndarray] = None, log: Optional[Callable[[str], None]] = None
+           ) -> FitResult:
+    """Plain Monte Carlo DE: every candidate simulated with sims clusters of n electrons."""
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    de = DifferentialEvolution(N_FIT, pop, rng, init=init)
+    key = DEFAULT_SEED + 1000 * seed
+    de.seed_fitness(objective(_mc_eval(ctx, full_theta(de.x, beta), sims, n, key), y, scale))
+    hist = [de.best[1]]
+    for g in range(gens):
+        trial = de.propose()
+        f = objective(_mc_eval(ctx, full_theta(trial, beta), sims, n, key + g + 1), y, scale)
+        de.accept(trial, f)
+        hist.append(de.best[1])
+        if log:
+            log(f"mc gen {g + 1}: best {hist[-1]:.3e}")
+    u, fbest = de.best
+    return FitResult(full_theta(u, beta)[0], fbest, float("nan"), time.time() - t0,
+                     pop * (gens + 1), 0, history=hist)
+
+
+def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, beta: float,
+                  pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
+                  fallback_pop: int = 64, fallback_gens: int = 20,
+                  log: Optional[Callable[[str], None]] = None) -> FitResult:
+    """Network-scored DE, Monte Carlo judge of the best candidate, Monte Carlo
+    fallback when the network misses the judge by more than gate (NRMSE)."""
+    import torch
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    de = DifferentialEvolution(N_FIT, pop, rng)
+
+    def net_eval(u5):
+        th = full_theta(u5, beta)
+        x = torch.from_numpy(S.encode(th)).cuda()
+        yq = net.forward(x).cpu().numpy()
+        return objective(S.denormalise(yq, net.c0), y, scale)
+
+    de.seed_fitness(net_eval(de.x))
+    hist = [de.best[1]]
+    for g in range(gens):
+        trial = de.propose()
+        de.accept(trial, net_eval(trial))
+        hist.append(de.best[1])
+    u, _ = de.best
+    th = full_theta(u, beta)
+    judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
+    pred = S.predict_curves(net, th)[0]
+    err = float(S.nrmse(pred, judge)[0])
+    if log:
+        log(f"surrogate fit: best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
+    n_net = pop * (gens + 1)
+    if err <= gate:
+        return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
+                         1, n_net, False, hist)
+    # fallback: Monte Carlo generations from the network's best candidates
+    order = np.argsort(de.f)[:fallback_pop]
+    res = fit_mc(ctx, y, scale, beta, pop=fallback_pop, gens=fallback_gens, seed=seed + 1,
+                 init=de.x[order], log=log)
+    return FitResult(res.theta, res.objective_mc, err, time.time() - t0,
+                     1 + res.mc_candidates, n_net, True, hist + res.history)

@@ -1,0 +1,101 @@
+"""Surrogate-assisted against plain Monte Carlo parameter fits on synthetic lab curves.
+
+    python -m mcluminescence_amd.fit_bench --targets 4 --model mcluminescence_amd/surrogate_tl.npz
+
+For each target: hidden parameters drawn inside the surrogate's box (heating rate
+1 K/s), a measured curve from the Monte Carlo with Poisson noise (peak 1e4 counts),
+then fit_surrogate and fit_mc on the same curve.  Prints one JSON line with the
+wall time of each fit, fits/hour, the judge NRMSE, how often the fallback ran,
+each fit's objective on a fresh Monte Carlo judge curve (beside the true
+parameters' own) and the parameter recovery errors of both methods.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+from pathlib import Path
+
+import numpy as np
+
+from . import fit as F
+from . import surrogate as S
+from ._abi import Context
+
+MODEL = Path(__file__).resolve().parent / "surrogate_tl.npz"
+
+
+def draw_truth(rng: np.random.Generator, beta: float = 1.0) -> np.ndarray:
+    """Hidden parameters away from the box edges (encoded coordinates in [-0.7, 0.7]);
+    fill fraction in [0.3, 1]."""
+    u = rng.uniform(-0.7, 0.7, S.N_IN)
+    th = S.decode(u[None])[0]
+    th[4] = rng.uniform(0.3, 1.0)
+    th[5] = beta
+    return th
+
+
+def run(targets: int = 4, model: Path = MODEL, mc_pop: int = 128, mc_gens: int = 40,
+        sur_pop: int = 1024, sur_gens: int = 150, seed: int = 0, log=print) -> dict:
+    ctx = Context(0)
+    net = S.Network.load(ctx, model, max_batch=max(sur_pop, 4096))
+    rng = np.random.default_rng(seed)
+    rows = []
+    for k in range(targets):
+        truth = draw_truth(rng)
+        y, scale = F.lab_curve(ctx, truth, rng=rng, seed=S.DEFAULT_SEED + 500 + k)
+        rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=sur_pop, gens=sur_gens, seed=k)
+        rm = F.fit_mc(ctx, y, scale, truth[5], pop=mc_pop, gens=mc_gens, seed=k)
+        row = {"truth": truth.tolist(),
+               "truth_objective": F.judge_objective(ctx, truth, y, scale),
+               "surrogate": {"seconds": rs.seconds,
+                             "objective": F.judge_objective(ctx, rs.theta, y, scale),
+                             "judge_nrmse": rs.judge_nrmse, "fallback": rs.fallback,
+                             "mc_candidates": rs.mc_candidates, "net_candidates": rs.net_candidates,
+                             "recovery": rs.recovery(truth)},
+               "mc": {"seconds": rm.seconds, "objective": F.judge_objective(ctx, rm.theta, y, scale),
+                      "mc_candidates": rm.mc_candidates, "recovery": rm.recovery(truth)}}
+        if log:
+            log(json.dumps(row))
+        rows.append(row)
+    ts = np.array([r["surrogate"]["seconds"] for r in rows])
+    tm = np.array([r["mc"]["seconds"] for r in rows])
+
+    def med(method, key):
+        return float(np.median([r[method]["recovery"][key] for r in rows]))
+
+    out = {
+        "targets": targets, "model": str(model),
+        "surrogate_fit_s": float(ts.mean()), "mc_fit_s": float(tm.mean()),
+        "speedup": float(tm.mean() / ts.mean()),
+        "surrogate_fits_per_hour": float(3600 / ts.mean()), "mc_fits_per_hour": float(3600 / tm.mean()),
+        "judge_nrmse_median": float(np.median([r["surrogate"]["judge_nrmse"] for r in rows])),
+        "fallback_rate": float(np.mean([r["surrogate"]["fallback"] for r in rows])),
+        "objective_surrogate_median": float(np.median([r["surrogate"]["objective"] for r in rows])),
+        "objective_mc_median": float(np.median([r["mc"]["objective"] for r in rows])),
+        "objective_truth_median": float(np.median([r["truth_objective"] for r in rows])),
+        "recovery_median": {m: {k: med(m, k) for k in ("E", "log10_s", "log10_rho_prime", "fill")}
+                            for m in ("surrogate", "mc")},
+        "mc_config": {"pop": mc_pop, "gens": mc_gens},
+        "surrogate_config": {"pop": sur_pop, "gens": sur_gens},
+    }
+    net.close()
+    ctx.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--targets", type=int, default=4)
+    ap.add_argument("--model", type=Path, default=MODEL)
+    ap.add_argument("--mc-pop", type=int, default=128)
+    ap.add_argument("--mc-gens", type=int, default=40)
+    ap.add_argument("--sur-pop", type=int, default=1024)
+    ap.add_argument("--sur-gens", type=int, default=150)
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args(argv)
+    out = run(a.targets, a.model, a.mc_pop, a.mc_gens, a.sur_pop, a.sur_gens, a.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

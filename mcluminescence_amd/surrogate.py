@@ -6,7 +6,8 @@ parameters to a 256-bin thermoluminescence glow curve:
   inputs   E [eV], s [1/s], alpha [1/m], rho [1/m^3] (hole density), fill fraction
            (e_ratio_start), heating rate beta [K/s]
   output   recombinations per temperature bin between 300 K and 750 K, as a fraction
-           of N_e, log-normalised: y = ln(1 + c / C0) / ln(1 + 1 / C0), C0 = 1e-4
+           of N_e, log-normalised: y = ln(1 + c / c0) / ln(1 + 1 / c0) (c0 = 1e-3 by
+           default, stored with the network)
 
 with the single-trap physics of ``workloads.single_trap`` (E_cb = E_loc = E,
 b = s).  Training curves come from the fixed-dt Monte Carlo (mcl_run_fixed_dt);
@@ -34,8 +35,7 @@ N_BINS = 256
 T_LO, T_HI = 300.0, 750.0          # K: glow-curve window
 STEPS_PER_BIN = 8
 N_STEPS = N_BINS * STEPS_PER_BIN
-C0 = 1e-4                          # log-normalisation floor
-_LOGN = math.log1p(1.0 / C0)
+C0 = 1e-3                          # default log-normalisation floor
 
 # parameter box: (name, low, high, log-scaled)
 RANGES = (
@@ -76,12 +76,12 @@ def sample_params(n: int, rng: np.random.Generator) -> np.ndarray:
     return decode(rng.uniform(-1.0, 1.0, size=(n, N_IN)))
 
 
-def normalise(c: np.ndarray) -> np.ndarray:
-    return (np.log1p(np.asarray(c, np.float64) / C0) / _LOGN).astype(np.float32)
+def normalise(c: np.ndarray, c0: float = C0) -> np.ndarray:
+    return (np.log1p(np.asarray(c, np.float64) / c0) / math.log1p(1.0 / c0)).astype(np.float32)
 
 
-def denormalise(y: np.ndarray) -> np.ndarray:
-    return C0 * np.expm1(np.asarray(y, np.float64) * _LOGN)
+def denormalise(y: np.ndarray, c0: float = C0) -> np.ndarray:
+    return c0 * np.expm1(np.asarray(y, np.float64) * math.log1p(1.0 / c0))
 
 
 def nrmse(pred: np.ndarray, ref: np.ndarray) -> np.ndarray:
@@ -136,8 +136,9 @@ class Network:
 
     def __init__(self, ctx: Context, n_in: int = N_IN, n_hidden: int = 512,
                  n_hidden_layers: int = 4, n_out: int = N_BINS, max_batch: int = 4096,
-                 params: Optional[np.ndarray] = None, seed: int = 0):
+                 params: Optional[np.ndarray] = None, seed: int = 0, c0: float = C0):
         self.ctx = ctx
+        self.c0 = float(c0)
         self.lib = ctx.lib
         self.desc = MlpDesc(n_in, n_hidden, n_hidden_layers, n_out, max_batch)
         h = C.c_void_p()
@@ -236,13 +237,15 @@ class Network:
     def save(self, path: Path) -> None:
         np.savez_compressed(path, params=self.get_params(),
                             desc=np.array([self.desc.n_in, self.desc.n_hidden,
-                                           self.desc.n_hidden_layers, self.desc.n_out]))
+                                           self.desc.n_hidden_layers, self.desc.n_out]),
+                            c0=np.array(self.c0))
 
     @staticmethod
     def load(ctx: Context, path: Path, max_batch: int = 65536) -> "Network":
         z = np.load(path)                      # plain arrays: no pickle
         n_in, n_hidden, n_layers, n_out = (int(v) for v in z["desc"])
-        return Network(ctx, n_in, n_hidden, n_layers, n_out, max_batch, params=z["params"])
+        return Network(ctx, n_in, n_hidden, n_layers, n_out, max_batch, params=z["params"],
+                       c0=float(z["c0"]))
 
 
 @dataclass
@@ -306,7 +309,7 @@ def predict_curves(net: Network, theta: np.ndarray) -> np.ndarray:
     out = []
     for c0 in range(0, x.shape[0], net.desc.max_batch):
         out.append(net.forward(x[c0:c0 + net.desc.max_batch]).cpu().numpy())
-    return denormalise(np.concatenate(out))
+    return denormalise(np.concatenate(out), net.c0)
 
 
 __all__ = ["Network", "TrainConfig", "train", "encode", "decode", "sample_params", "normalise",

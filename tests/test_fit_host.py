@@ -1,0 +1,73 @@
+"""Host logic of the surrogate and the fit loop (no GPU): parameter encoding,
+curve normalisation, the NRMSE and objective definitions, differential evolution."""
+import numpy as np
+import pytest
+
+from mcluminescence_amd import fit as F
+from mcluminescence_amd import surrogate as S
+from mcluminescence_amd.workloads import rho_prime_for
+
+
+def test_encode_decode_round_trip_and_box():
+    rng = np.random.default_rng(0)
+    th = S.sample_params(1000, rng)
+    assert np.all(th >= S.LOW * (1 - 1e-12)) and np.all(th <= S.HIGH * (1 + 1e-12))
+    u = S.encode(th)
+    assert np.all(np.abs(u) <= 1 + 1e-6)
+    np.testing.assert_allclose(S.decode(u), th, rtol=1e-6)
+    # log axes: the box centre of E is the geometric mean
+    assert abs(S.decode(np.zeros((1, 6)))[0, 0] - np.sqrt(1.0 * 1.8)) < 1e-12
+
+
+@pytest.mark.parametrize("c0", [1e-3, 1e-4])
+def test_normalise_round_trip(c0):
+    c = np.array([0.0, 1e-6, 1e-4, 0.01, 0.3, 1.0])
+    y = S.normalise(c, c0)
+    assert y[0] == 0 and abs(y[-1] - 1) < 1e-6 and np.all(np.diff(y) > 0)
+    np.testing.assert_allclose(S.denormalise(y, c0), c, rtol=1e-5, atol=1e-9)
+
+
+def test_nrmse_definition():
+    ref = np.array([[0.0, 1.0, 4.0, 1.0]])
+    assert S.nrmse(ref, ref)[0] == 0
+    pred = ref + np.array([[0.0, 0.0, 0.4, 0.0]])
+    assert abs(S.nrmse(pred, ref)[0] - np.sqrt(0.16 / 4) / 4.0) < 1e-15
+
+
+def test_objective_is_squared_nrmse_against_the_measured_curve():
+    c = np.array([0.1, 0.5, 0.2])
+    y = c * 1000.0
+    assert F.objective(c, y, 1000.0)[0] == 0
+    off = F.objective(c * 1.1, y, 1000.0)[0]
+    assert abs(np.sqrt(off) - S.nrmse(c * 1.1 * 1000, y)[0]) < 1e-12
+
+
+def test_full_theta_fixes_the_heating_rate():
+    th = F.full_theta(np.zeros(5), 2.5)
+    assert th.shape == (1, 6) and th[0, 5] == 2.5
+
+
+def test_differential_evolution_finds_a_quadratic_minimum():
+    rng = np.random.default_rng(1)
+    de = F.DifferentialEvolution(5, 48, rng)
+    target = np.array([0.3, -0.2, 0.5, 0.1, -0.6])
+    f = lambda x: ((x - target) ** 2).sum(axis=1)
+    de.seed_fitness(f(de.x))
+    for _ in range(150):
+        trial = de.propose()
+        assert np.all(np.abs(trial) <= 1)
+        de.accept(trial, f(trial))
+    x, fx = de.best
+    assert fx < 1e-10
+    np.testing.assert_allclose(x, target, atol=1e-5)
+
+
+def test_recovery_uses_rho_prime():
+    th = np.array([1.3, 1e12, 4e9, 1e23, 0.5, 1.0])
+    # same rho' with another alpha: no rho' error
+    th2 = th.copy()
+    th2[2] = 5e9
+    th2[3] = 1e23 * (5e9 / 4e9) ** 3
+    r = F.FitResult(th2, 0.0, 0.0, 0.0, 0, 0).recovery(th)
+    assert abs(rho_prime_for(th2[3], th2[2]) / rho_prime_for(th[3], th[2]) - 1) < 1e-12
+    assert r["log10_rho_prime"] < 1e-12 and r["E"] == 0
