@@ -476,7 +476,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const int p0 = r * kWave + lane, p1 = p0 + kWave;
             const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
             const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
-            const float w0 = ew[p0], w1 = ew[p1];
+            const float w0 = p0 < n_slots ? ew[p0] : 0.0f, w1 = p1 < n_slots ? ew[p1] : 0.0f;
             const bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);

@@ -201,7 +201,7 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
             max_cells = std::max(max_cells, d.grid * d.grid * d.grid);
         }
         cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
-        cl.e_cap = ((cl.max_e + 63) / 64) * 64 + 64;
+        cl.e_cap = std::max(64, ((cl.max_e + 63) / 64) * 64);
         cl.g_cap = ((max_cells + 1 + 3) / 4) * 4;
         cl.lds = mcl::fixed_dt_lds_bytes(cl.h_cap, cl.e_cap, cl.g_cap);
         if (cl.lds > c->max_lds)
@@ -262,7 +262,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
     if (total > 0x7FFFFFFF) return fail(c, MCL_EINVAL, "too many clusters (%lld)", (long long)total);
     const int h_cap = ((max_h + 3) / 4) * 4 + 4;
-    const int e_cap = ((max_e + 63) / 64) * 64 + 64;
+    const int e_cap = std::max(64, ((max_e + 63) / 64) * 64);
     const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
     const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
     if (lds > c->max_lds)
@@ -481,8 +481,8 @@ int mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_hol
     const Ctx* c = (const Ctx*)ctx;
     if (!c || !max_electrons || !max_holes) return MCL_EINVAL;
     // equal electron and hole counts, about 2 holes per grid cell:
-    // 16 (h + 4) + 4 (h / 2 + 4) + 16 (e + 64) <= max_lds
-    int n = (c->max_lds - 16 * 4 - 4 * 4 - 16 * 64) / 34;
+    // 16 (h + 4) + 4 (h / 2 + 4) + 16 e <= max_lds
+    int n = (c->max_lds - 16 * 4 - 4 * 4) / 34;
     n = (n / 64) * 64;
     *max_electrons = n > 65535 ? 65535 : n;
     *max_holes = n > 65534 ? 65534 : n;

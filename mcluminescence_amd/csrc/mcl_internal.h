@@ -55,7 +55,7 @@ struct LaunchArgs {
     uint64_t* work;         // [n_clusters][4] or null: live electron-steps, distance
                             // evaluations, recombinations, steps simulated
     int32_t h_cap;          // LDS hole capacity (multiple of 4)
-    int32_t e_cap;          // LDS electron capacity (multiple of 64)
+    int32_t e_cap;          // LDS electron capacity (multiple of 64, >= n_e0)
     int32_t g_cap;          // LDS cell-start capacity (>= G^3 + 1, multiple of 4)
     int32_t wave_search_max;  // queues up to this long are searched one electron per wave (<= 64)
     StateDump dump;

@@ -165,35 +165,48 @@ def fit_mc(ctx: Context, y: np.ndarray, scale: float, beta: float, pop: int = 64
 
 def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, beta: float,
                   pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
-                  fallback_pop: int = 64, fallback_gens: int = 20,
+                  fallback_pop: int = 64, fallback_gens: int = 20, patience: int = 25,
                   log: Optional[Callable[[str], None]] = None) -> FitResult:
     """Network-scored DE, Monte Carlo judge of the best candidate, Monte Carlo
-    fallback when the network misses the judge by more than gate (NRMSE)."""
+    fallback when the network misses the judge by more than gate (NRMSE).  The
+    search stops early once the best objective has not improved by 1e-4 of itself
+    for `patience` generations."""
     import torch
     t0 = time.time()
     rng = np.random.default_rng(seed)
     de = DifferentialEvolution(N_FIT, pop, rng)
+    # the encoded fit coordinates are the network's inputs; beta is the sixth
+    u_beta = float(S.encode(full_theta(np.zeros(N_FIT), beta))[0, 5])
+    y_dev = torch.from_numpy(np.asarray(y, np.float32)).cuda()
+    inv_peak2 = 1.0 / max(float(y.max()), 1.0) ** 2
+    log_span = math.log1p(1.0 / net.c0)
 
     def net_eval(u5):
-        th = full_theta(u5, beta)
-        x = torch.from_numpy(S.encode(th)).cuda()
-        yq = net.forward(x).cpu().numpy()
-        return objective(S.denormalise(yq, net.c0), y, scale)
+        x = np.empty((u5.shape[0], S.N_IN), np.float32)
+        x[:, :N_FIT] = u5
+        x[:, N_FIT] = u_beta
+        yq = net.forward(torch.from_numpy(x).cuda())
+        d = torch.expm1(yq * log_span) * (net.c0 * scale) - y_dev     # objective() on device
+        return ((d * d).mean(dim=1) * inv_peak2).double().cpu().numpy()
 
     de.seed_fitness(net_eval(de.x))
     hist = [de.best[1]]
+    stall = 0
     for g in range(gens):
         trial = de.propose()
         de.accept(trial, net_eval(trial))
         hist.append(de.best[1])
+        stall = stall + 1 if hist[-1] > hist[-2] * (1.0 - 1e-4) else 0
+        if stall >= patience:
+            break
     u, _ = de.best
     th = full_theta(u, beta)
     judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
     pred = S.predict_curves(net, th)[0]
     err = float(S.nrmse(pred, judge)[0])
     if log:
-        log(f"surrogate fit: best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
-    n_net = pop * (gens + 1)
+        log(f"surrogate fit: {len(hist) - 1} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
+    n_net = pop * len(hist)
     if err <= gate:
         return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
                          1, n_net, False, hist)
