@@ -41,8 +41,13 @@
 //   conflict  ds_min_u32 of the electron index into the hole's claim field; the
 //             lowest index keeps the hole (src/est_params/functions.py:228-239).
 //   remove    winners' slots become dead, their holes get the x sentinel.
-//   repair    an electron whose hole was removed is searched again at the start
-//             of the next step, before its fire test (a conflict loser at once).
+//   repair    an electron whose hole was removed keeps its old w until a step in
+//             which its draw would fire against that old w: only then is it
+//             searched again (before its fire test; a conflict loser at once).
+//             Exact: the new nearest hole is no closer, so w and x = k dt can only
+//             fall, and the threshold of x is non-decreasing in x (checked for
+//             every float x by the oracle), so a draw that fails against the old
+//             w fails against the new one.
 //   retry     a conflict loser retries on its new nearest hole within the step,
 //             accepted with probability T_new / T_old (one more Philox word), for
 //             up to MCL_MAX_ROUNDS rounds; this removes the O(k dt) delay a
@@ -391,6 +396,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     auto claim = [&](uint32_t nn, uint32_t o) {
         atomicMin(&hole_u[4 * nn + 3], (o << 16) | (hole_u[4 * nn + 3] & 0xFFFFu));
     };
+    // Would draw u fire with x = k dt?
+    auto may_fire = [&](uint32_t u, float x) {
+        FireThr t = tA;
+        if (x != xA) t = fire_threshold(x);
+        return t.always || u < t.thr;
+    };
     // Test of one live electron (slot p) with draw u and x = k dt; candidates claim
     // their hole and remember their threshold for a conflict retry.
     auto fire_tail = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
@@ -467,9 +478,10 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         const int rows = (n_slots + kWave - 1) / kWave;
 
         // Pass A, two rows per iteration: an electron whose nearest hole was removed
-        // in the previous step is queued for a new search (its test waits for it);
-        // every other live electron takes its fire test now.  Deferring that search
-        // to here changes nothing: no hole disappears in between.
+        // and whose draw would fire against its old w is queued for a new search
+        // (its test waits for it); one whose draw fails even against the old w stays
+        // as it is; every other live electron takes its fire test now.  Deferring
+        // the search to here changes nothing: no hole disappears in between.
         uint64_t cand = 0, stale = 0;
         int nq = 0;
         for (int r = 0; r < rows; r += 2) {
@@ -477,7 +489,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
             const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
             const float w0 = p0 < n_slots ? ew[p0] : 0.0f, w1 = p1 < n_slots ? ew[p1] : 0.0f;
-            const bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
+            bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);
             const uint32_t u1 = pick_word(philox10(blk, o1, prun, c3f, k0, k1), word);
@@ -486,8 +498,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                 ch0 = (uint64_t)pick_word(philox10(blk, o0, prun, c3c, k0, k1), word) < ret_thr;
                 ch1 = (uint64_t)pick_word(philox10(blk, o1, prun, c3c, k0, k1), word) < ret_thr;
             }
-            if (!s0) fire_tail(st0, o0, p0, u0, __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA), r, cand);
-            if (!s1) fire_tail(st1, o1, p1, u1, __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA), r + 1, cand);
+            const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
+            const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
+            if (!s0) fire_tail(st0, o0, p0, u0, x0, r, cand);
+            else s0 = may_fire(u0, x0);
+            if (!s1) fire_tail(st1, o1, p1, u1, x1, r + 1, cand);
+            else s1 = may_fire(u1, x1);
             const uint64_t m0 = ballot(s0);
             if (s0) rq[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
             nq += __popcll(m0);

@@ -564,3 +564,28 @@ void orc_bruteforce_nn(const float* e_xyz, int n_e, const float* h_xyz, const un
         nn[e] = idx;
     }
 }
+
+/* Monotonicity of the float32 fire threshold: walks every float x in [lo, hi]
+ * in increasing order and counts the places where T(x) falls below T at a
+ * smaller x (the running maximum).  *worst gets the largest such drop. */
+uint64_t orc_thr_monotone_scan(float lo, float hi, uint64_t* worst)
+{
+    uint64_t bad = 0, run_max = 0;
+    *worst = 0;
+    uint32_t b = 0, e = 0;
+    memcpy(&b, &lo, 4);
+    memcpy(&e, &hi, 4);
+    for (uint32_t i = b; i <= e; ++i) {
+        float x;
+        memcpy(&x, &i, 4);
+        const uint64_t t = thr_f32(x);
+        if (t < run_max) {
+            ++bad;
+            if (run_max - t > *worst) *worst = run_max - t;
+        } else {
+            run_max = t;
+        }
+        if (i == 0xFFFFFFFFu) break;
+    }
+    return bad;
+}

@@ -69,6 +69,17 @@ def expf(x: float) -> np.float32:
     return np.float32(lib().orc_expf(float(x)))
 
 
+def thr_monotone_scan(lo: float, hi: float):
+    """(violations, largest drop) of the float32 fire threshold over every float
+    in [lo, hi] taken in increasing order."""
+    L = lib()
+    L.orc_thr_monotone_scan.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_uint64)]
+    L.orc_thr_monotone_scan.restype = C.c_uint64
+    worst = C.c_uint64()
+    bad = L.orc_thr_monotone_scan(float(lo), float(hi), C.byref(worst))
+    return int(bad), int(worst.value)
+
+
 def exp64(x: float) -> float:
     return float(lib().orc_exp(float(x)))
 

@@ -58,3 +58,11 @@ def test_special_values():
         assert orc.expm1f(-x) == np.float32(-x)
     # at 1e-7, x^2/2 is 0.7 ulp of x: within one ulp
     assert abs(float(orc.expm1f(-1e-7)) - math.expm1(float(np.float32(-1e-7)))) <= _ulp32(1e-7)
+
+
+def test_fire_threshold_is_monotone_for_every_float():
+    # The kernel skips the nearest-hole search of an electron whose draw fails
+    # against its old (closer) hole: exact only if T(x) never decreases as x
+    # grows.  Every float in [0, 30] (x >= 17.5 already gives p = 1).
+    bad, worst = orc.thr_monotone_scan(0.0, 30.0)
+    assert (bad, worst) == (0, 0)
