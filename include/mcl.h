@@ -114,6 +114,9 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *   "wave_search_max" electrons needing a new nearest hole in one step are
  *                     searched one at a time by the whole wave up to this
  *                     many, lane-parallel beyond (default 2, at most 64).
+ *   "lookahead_max"   with at most this many live electrons in a cluster, the
+ *                     wave tests the next 64 steps at once and skips those in
+ *                     which no electron can fire (default 8; 0 turns it off).
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

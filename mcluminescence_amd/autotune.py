@@ -6,6 +6,8 @@ The kernels' results do not depend on the tuning knobs (mcl_tune_set):
                    per lookup against the rate of inconclusive 3x3x3 searches)
   wave_search_max  queue length up to which a step's nearest-hole searches run one
                    electron per wave rather than lane-parallel
+  lookahead_max    live electrons up to which a cluster tests 64 steps at once and
+                   skips the quiet ones
   persist_frac     persistent workgroups per size class, as a fraction of what
                    fits on the device at once (load balance against contention
                    between concurrently running class kernels)
@@ -27,9 +29,10 @@ from typing import Dict, Iterable, List, Optional, Sequence
 from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geometry
 
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
-DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 2.0}
+DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 2.0,
+            "lookahead_max": 8.0}
 SPACE = {"holes_per_cell": [2.0, 3.0, 4.0], "persist_frac": [1.0, 2.0],
-         "wave_search_max": [0.0, 2.0, 4.0]}
+         "wave_search_max": [0.0, 2.0, 4.0], "lookahead_max": [0.0, 8.0, 32.0]}
 
 
 def signature(runs: Sequence[RunParams]) -> str:

@@ -457,25 +457,103 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         }
     };
 
-    for (int n = 0; n < step_limit; ++n) {
-        if (n_live == 0) break;
+    // Step scalars of step m: start-of-step temperature, Boltzmann factors in
+    // float64, their float32 products with dt, the threshold of the band term.
+    struct StepScalars {
+        float xA, b1dt, b2dt;
+        FireThr tA;
+    };
+    auto scalars_at = [&](int m) {
+        const double t = (double)m * R.dt;
+        const double T = R.T0_K + R.T_rate * t;
+        const double kT = R.k_b * T;
+        const double A = R.s * exp_own(-(R.E_cb / kT));
+        const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
+        const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
+        StepScalars sc;
+        sc.xA = (float)(A * R.dt);
+        sc.b1dt = (float)(B1 * R.dt);
+        sc.b2dt = (float)(B2 * R.dt);
+        sc.tA = fire_threshold(sc.xA);
+        return sc;
+    };
+    // Look-ahead for few live electrons: lane j tests step n + j for every live
+    // electron with exactly the draw and threshold that step would use (a stale
+    // electron against its old w, as in pass A).  Returns how many steps from n on
+    // no electron can fire (at most 64, up to step_limit); found tells whether
+    // some step of the window may fire.  Such quiet steps change nothing but
+    // the live-step and bin tallies.
+    auto quiet_steps = [&](int n0, bool& found) -> int {
+        const int len = min(kWave, step_limit - n0);
+        const int m = n0 + lane;
+        StepScalars sc{xA, b1dt, b2dt, tA};
+        if (R.T_rate != 0.0) sc = scalars_at(m);
+        const uint32_t mblk = (uint32_t)m >> 2;
+        const int mword = m & 3;
+        bool fire = false;
+        for (int p = 0; p < n_slots; ++p) {
+            const uint32_t st = est[p];                  // one address: a broadcast read
+            if (st == kDeadSlot || (st >> 16) == kNoHole) continue;
+            const uint32_t o = st & 0xFFFFu;
+            const float w = ew[p];
+            const uint32_t u = pick_word(philox10(mblk, o, prun, c3f, k0, k1), mword);
+            bool ch = always2;
+            if (two_ch)
+                ch = (uint64_t)pick_word(philox10(mblk, o, prun, c3c, k0, k1), mword) < ret_thr;
+            const float x = __builtin_fmaf(ch ? sc.b2dt : sc.b1dt, w, sc.xA);
+            FireThr t = sc.tA;
+            if (x != sc.xA) t = fire_threshold(x);
+            fire = fire || t.always || u < t.thr;
+        }
+        const uint64_t fm = ballot(fire && lane < len);
+        found = fm != 0;
+        return found ? (int)__builtin_ctzll(fm) : len;
+    };
+    // Bin tally at the end of a step with `wins` recombinations.
+    auto finish_step = [&](uint32_t wins) {
+        bin_acc += wins;
+        bin_rem += nb;
+        if (bin_rem >= n_steps) {
+            bin_rem -= n_steps;
+            if (lane == 0) cnt[bin] = bin_acc;
+            bin_acc = 0;
+            ++bin;
+        }
+    };
+
+    {
+        const StepScalars sc = scalars_at(0);
+        xA = sc.xA;
+        b1dt = sc.b1dt;
+        b2dt = sc.b2dt;
+        tA = sc.tA;
+    }
+    int n = 0;
+    while (n < step_limit && n_live > 0) {
+        if (n_live <= a.lookahead_max) {
+            bool found = false;
+            const int f = quiet_steps(n, found);
+            for (int k = 0; k < f; ++k) {
+                lsteps += (uint64_t)n_live;
+                finish_step(0u);
+            }
+            n += f;
+            steps_run = n;
+            if (!found) continue;
+        }
         lsteps += (uint64_t)n_live;
         steps_run = n + 1;
-        if (n == 0 || R.T_rate != 0.0) {
-            const double t = (double)n * R.dt;
-            const double T = R.T0_K + R.T_rate * t;
-            const double kT = R.k_b * T;
-            const double A = R.s * exp_own(-(R.E_cb / kT));
-            const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
-            const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
-            xA = (float)(A * R.dt);
-            b1dt = (float)(B1 * R.dt);
-            b2dt = (float)(B2 * R.dt);
-            tA = fire_threshold(xA);
+        if (n > 0 && R.T_rate != 0.0) {
+            const StepScalars sc = scalars_at(n);
+            xA = sc.xA;
+            b1dt = sc.b1dt;
+            b2dt = sc.b2dt;
+            tA = sc.tA;
         }
         const uint32_t blk = (uint32_t)n >> 2;
         const int word = n & 3;
         const int rows = (n_slots + kWave - 1) / kWave;
+        const float bmax = fmaxf(b1dt, b2dt);
 
         // Pass A, two rows per iteration: an electron whose nearest hole was removed
         // and whose draw would fire against its old w is queued for a new search
@@ -491,6 +569,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const float w0 = p0 < n_slots ? ew[p0] : 0.0f, w1 = p1 < n_slots ? ew[p1] : 0.0f;
             bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
+            // Below x = 2^-32 (in either channel) the threshold is 0: no draw can
+            // fire, and a stale electron keeps waiting.  Rows whose lanes all sit
+            // there (early in a heating ramp) skip their draws.
+            if (ballot(__builtin_fmaf(bmax, w0, xA) >= 0x1p-32f ||
+                       __builtin_fmaf(bmax, w1, xA) >= 0x1p-32f) == 0)
+                continue;
             const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);
             const uint32_t u1 = pick_word(philox10(blk, o1, prun, c3f, k0, k1), word);
             bool ch0 = always2, ch1 = always2;
@@ -656,14 +740,8 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                 __syncthreads();
             }
         }
-        bin_acc += wins;
-        bin_rem += nb;
-        if (bin_rem >= n_steps) {
-            bin_rem -= n_steps;
-            if (lane == 0) cnt[bin] = bin_acc;
-            bin_acc = 0;
-            ++bin;
-        }
+        finish_step(wins);
+        ++n;
     }
     if (bin < nb) {
         if (lane == 0) cnt[bin] = bin_acc;

@@ -345,6 +345,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.e_cap = p->e_cap;
     a.g_cap = p->g_cap;
     a.wave_search_max = (int32_t)p->ctx->wave_search_max;
+    a.lookahead_max = (int32_t)p->ctx->lookahead_max;
     a.dump.enabled = 0;
     return a;
 }
@@ -505,6 +506,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->wave_search_max = std::floor(value);
         return MCL_OK;
     }
+    if (std::strcmp(key, "lookahead_max") == 0) {
+        if (!(value >= 0.0 && value <= 65535.0))
+            return fail(c, MCL_EINVAL, "lookahead_max must be in [0, 65535] (got %g)", value);
+        c->lookahead_max = std::floor(value);
+        return MCL_OK;
+    }
     if (std::strcmp(key, "persist_frac") == 0) {
         if (!(value > 0.0 && value <= 4.0))
             return fail(c, MCL_EINVAL, "persist_frac must be in (0, 4] (got %g)", value);
@@ -524,6 +531,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "persist_frac") == 0) {
         *value = c->persist_frac;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "lookahead_max") == 0) {
+        *value = c->lookahead_max;
         return MCL_OK;
     }
     if (std::strcmp(key, "wave_search_max") == 0) {

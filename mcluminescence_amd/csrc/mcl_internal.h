@@ -58,6 +58,7 @@ struct LaunchArgs {
     int32_t e_cap;          // LDS electron capacity (multiple of 64, >= n_e0)
     int32_t g_cap;          // LDS cell-start capacity (>= G^3 + 1, multiple of 4)
     int32_t wave_search_max;  // queues up to this long are searched one electron per wave (<= 64)
+    int32_t lookahead_max;    // with at most this many live electrons, quiet steps are skipped
     StateDump dump;
 };
 

@@ -589,3 +589,6 @@ uint64_t orc_thr_monotone_scan(float lo, float hi, uint64_t* worst)
     }
     return bad;
 }
+
+/* The float32 fire threshold T(x) in [0, 2^32] (2^32: always fires). */
+uint64_t orc_fire_threshold(float x) { return thr_f32(x); }

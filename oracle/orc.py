@@ -69,6 +69,14 @@ def expf(x: float) -> np.float32:
     return np.float32(lib().orc_expf(float(x)))
 
 
+def fire_threshold(x: float) -> int:
+    """T(x) = (uint32)((1 - e^{-x}) 2^32), or 2^32 when p rounds to 1."""
+    L = lib()
+    L.orc_fire_threshold.argtypes = [C.c_float]
+    L.orc_fire_threshold.restype = C.c_uint64
+    return int(L.orc_fire_threshold(float(x)))
+
+
 def thr_monotone_scan(lo: float, hi: float):
     """(violations, largest drop) of the float32 fire threshold over every float
     in [lo, hi] taken in increasing order."""

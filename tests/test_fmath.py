@@ -66,3 +66,11 @@ def test_fire_threshold_is_monotone_for_every_float():
     # grows.  Every float in [0, 30] (x >= 17.5 already gives p = 1).
     bad, worst = orc.thr_monotone_scan(0.0, 30.0)
     assert (bad, worst) == (0, 0)
+
+
+def test_fire_threshold_is_zero_below_two_to_minus_32():
+    # Rows whose x are all below 2^-32 skip their draws: T must be 0 there.  With
+    # T monotone (above), the largest float below 2^-32 settles every smaller x.
+    below = np.nextafter(np.float32(2.0 ** -32), np.float32(0.0))
+    assert orc.fire_threshold(float(below)) == 0
+    assert orc.fire_threshold(2.0 ** -31) > 0

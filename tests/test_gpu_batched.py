@@ -47,17 +47,20 @@ def test_graph_replays_match_eager(gpu_ctx):
 
 
 def test_tuning_knobs_do_not_change_results(gpu_ctx):
-    runs = _sweep()[:80]
-    base, _, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
+    runs = _sweep()[:80] + W.tl_glow_16x500(sims=4)
+    base, lb, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
     try:
-        for hpc, pf in [(1.0, 0.5), (4.0, 2.0)]:
+        for hpc, pf, wsm, la in [(1.0, 0.5, 0, 0), (4.0, 2.0, 64, 64), (2.0, 1.0, 2, 65535)]:
             gpu_ctx.tune_set("holes_per_cell", hpc)
             gpu_ctx.tune_set("persist_frac", pf)
-            got, _, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
+            gpu_ctx.tune_set("wave_search_max", wsm)
+            gpu_ctx.tune_set("lookahead_max", la)
+            got, lg, _ = gpu_ctx.run_fixed_dt(runs, n_bins=48, flags=F_PERSISTENT)
             np.testing.assert_array_equal(got, base)
+            np.testing.assert_array_equal(lg, lb)
     finally:
-        gpu_ctx.tune_set("holes_per_cell", 2.0)
-        gpu_ctx.tune_set("persist_frac", 1.0)
+        for k, v in autotune.DEFAULTS.items():
+            gpu_ctx.tune_set(k, v)
 
 
 def test_autotuner_picks_a_measured_point(gpu_ctx, tmp_path):
