@@ -133,6 +133,8 @@ def surrogate_extras(ctx) -> dict:
     rng = np.random.default_rng(7)
     truth = fit_bench.draw_truth(rng)
     y, scale = F.lab_curve(ctx, truth, rng=rng)
+    F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=2, seed=9)   # loads kernels
+    F.fit_mc(ctx, y, scale, truth[5], pop=8, gens=1, seed=9)
     rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=150, seed=1)
     rm = F.fit_mc(ctx, y, scale, truth[5], pop=128, gens=40, seed=1)
     net.close()

@@ -109,6 +109,53 @@ class DifferentialEvolution:
         return self.x[i].copy(), float(self.f[i])
 
 
+class DeviceDifferentialEvolution:
+    """The same DE/rand-to-best/1/bin step with the population kept in device
+    memory (torch tensors), so a generation of the network-scored search issues a
+    few asynchronous kernels and no host round trip.  Partners are three distinct
+    offsets from the target, so each target's partners differ from it and from
+    one another."""
+
+    def __init__(self, d: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3):
+        import torch
+        self.torch = torch
+        self.d, self.P, self.F, self.CR = d, pop, F, CR
+        self.g = torch.Generator(device="cuda").manual_seed(int(seed))
+        self.x = torch.rand((pop, d), generator=self.g, device="cuda", dtype=torch.float32) * 2 - 1
+        self.f = torch.full((pop,), float("inf"), device="cuda", dtype=torch.float64)
+        self.idx = torch.arange(pop, device="cuda")
+
+    def _randint(self, lo: int, hi: int):
+        return self.torch.randint(lo, hi, (self.P,), generator=self.g, device="cuda")
+
+    def propose(self):
+        t, P, d = self.torch, self.P, self.d
+        best = self.x[t.argmin(self.f)]
+        a = self._randint(1, P)
+        b = self._randint(1, P - 1)
+        b = b + (b >= a).long()
+        c = self._randint(1, P - 2)
+        lo, hi = t.minimum(a, b), t.maximum(a, b)
+        c = c + (c >= lo).long()
+        c = c + (c >= hi).long()
+        x0, x1, x2 = self.x[(self.idx + a) % P], self.x[(self.idx + b) % P], self.x[(self.idx + c) % P]
+        mutant = x0 + self.F * (best - x0) + self.F * (x1 - x2)
+        cross = t.rand((P, d), generator=self.g, device="cuda") < self.CR
+        cross[self.idx, t.randint(0, d, (P,), generator=self.g, device="cuda")] = True
+        trial = t.where(cross, mutant, self.x)
+        redraw = t.rand((P, d), generator=self.g, device="cuda", dtype=t.float32) * 2 - 1
+        return t.where((trial < -1) | (trial > 1), redraw, trial)
+
+    def accept(self, trial, f) -> None:
+        better = f <= self.f
+        self.x = self.torch.where(better[:, None], trial, self.x)
+        self.f = self.torch.where(better, f, self.f)
+
+    def best(self):
+        i = int(self.torch.argmin(self.f).item())
+        return self.x[i].double().cpu().numpy(), float(self.f[i].item())
+
+
 @dataclass
 class FitResult:
     theta: np.ndarray                 # physical parameters [6] (beta included)
@@ -167,52 +214,50 @@ def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, bet
                   pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
                   fallback_pop: int = 64, fallback_gens: int = 20, patience: int = 25,
                   log: Optional[Callable[[str], None]] = None) -> FitResult:
-    """Network-scored DE, Monte Carlo judge of the best candidate, Monte Carlo
-    fallback when the network misses the judge by more than gate (NRMSE).  The
-    search stops early once the best objective has not improved by 1e-4 of itself
-    for `patience` generations."""
+    """Network-scored DE (population on the device), Monte Carlo judge of the best
+    candidate, Monte Carlo fallback when the network misses the judge by more than
+    gate (NRMSE).  Every `patience` generations the best objective is read back;
+    the search stops once it has improved by less than 1e-4 of itself since."""
     import torch
     t0 = time.time()
-    rng = np.random.default_rng(seed)
-    de = DifferentialEvolution(N_FIT, pop, rng)
+    de = DeviceDifferentialEvolution(N_FIT, pop, seed)
     # the encoded fit coordinates are the network's inputs; beta is the sixth
     u_beta = float(S.encode(full_theta(np.zeros(N_FIT), beta))[0, 5])
     y_dev = torch.from_numpy(np.asarray(y, np.float32)).cuda()
     inv_peak2 = 1.0 / max(float(y.max()), 1.0) ** 2
     log_span = math.log1p(1.0 / net.c0)
+    beta_col = torch.full((pop, 1), u_beta, device="cuda")
 
     def net_eval(u5):
-        x = np.empty((u5.shape[0], S.N_IN), np.float32)
-        x[:, :N_FIT] = u5
-        x[:, N_FIT] = u_beta
-        yq = net.forward(torch.from_numpy(x).cuda())
+        yq = net.forward(torch.cat([u5, beta_col], dim=1))
         d = torch.expm1(yq * log_span) * (net.c0 * scale) - y_dev     # objective() on device
-        return ((d * d).mean(dim=1) * inv_peak2).double().cpu().numpy()
+        return ((d * d).mean(dim=1) * inv_peak2).double()
 
-    de.seed_fitness(net_eval(de.x))
-    hist = [de.best[1]]
-    stall = 0
-    for g in range(gens):
-        trial = de.propose()
-        de.accept(trial, net_eval(trial))
-        hist.append(de.best[1])
-        stall = stall + 1 if hist[-1] > hist[-2] * (1.0 - 1e-4) else 0
-        if stall >= patience:
+    de.accept(de.x, net_eval(de.x))
+    hist = [float(de.f.min().item())]
+    g = 0
+    while g < gens:
+        for _ in range(min(patience, gens - g)):
+            trial = de.propose()
+            de.accept(trial, net_eval(trial))
+            g += 1
+        hist.append(float(de.f.min().item()))
+        if hist[-1] > hist[-2] * (1.0 - 1e-4):
             break
-    u, _ = de.best
+    u, _ = de.best()
     th = full_theta(u, beta)
     judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
     pred = S.predict_curves(net, th)[0]
     err = float(S.nrmse(pred, judge)[0])
     if log:
-        log(f"surrogate fit: {len(hist) - 1} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
-    n_net = pop * len(hist)
+        log(f"surrogate fit: {g} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
+    n_net = pop * (g + 1)
     if err <= gate:
         return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
                          1, n_net, False, hist)
     # fallback: Monte Carlo generations from the network's best candidates
-    order = np.argsort(de.f)[:fallback_pop]
+    order = torch.argsort(de.f)[:fallback_pop]
     res = fit_mc(ctx, y, scale, beta, pop=fallback_pop, gens=fallback_gens, seed=seed + 1,
-                 init=de.x[order], log=log)
+                 init=de.x[order].double().cpu().numpy(), log=log)
     return FitResult(res.theta, res.objective_mc, err, time.time() - t0,
                      1 + res.mc_candidates, n_net, True, hist + res.history)
