@@ -37,6 +37,9 @@ PEAK_VALU_LANE_SLOTS = 256 * 64 * 2.4e9
 PEAK_HBM_GBPS = 8000.0
 SLOTS_PER_LIVE_ESTEP = 28.0     # 1/4 x (20 v_mad_u64_u32 x 4 + 20 v_xor3) + fma/cmp/select
 SLOTS_PER_DISTANCE = 9.0        # 3 sub, mul, 2 fma, cmp, 2 cndmask
+# HBM bytes per launch of the decay workload, from rocprofv3 FETCH_SIZE (590.3 KiB,
+# x2 by a plain-copy calibration) and WRITE_SIZE (64,260.6 KiB, x1)
+TRAFFIC_BYTES_DECAY = (590.3 * 2 + 64260.6) * 1024
 
 
 def cpu_baseline(seconds: float) -> dict:
@@ -99,10 +102,10 @@ def tl_extras(ctx, stream) -> dict:
                               "1024 parameter sets per call, host round trip included"}
 
 
-def surrogate_extras(ctx) -> dict:
+def surrogate_extras(ctx, fits: bool = True) -> dict:
     """Surrogate network: one AdamW training step at batch 4096 and one inference
-    batch of 65,536 (device time, HIP events), and one surrogate-assisted fit of a
-    synthetic lab curve against the plain Monte Carlo fit of the same curve."""
+    batch of 65,536 (device time, HIP events), and (fits) one surrogate-assisted fit
+    of a synthetic lab curve against the plain Monte Carlo fit of the same curve."""
     import numpy as np
     import torch
     from mcluminescence_amd import fit as F
@@ -130,6 +133,14 @@ def surrogate_extras(ctx) -> dict:
         tr.append(net.last_ms())
     net.set_params(params)
     inf_ms, tr_ms = float(np.median(inf)), float(np.median(tr[3:]))
+    timing = {"inference_b65536_ms": inf_ms,
+              "inference_tflops": 65536 * flop / (inf_ms * 1e-3) / 1e12,
+              "inference_frac_bf16_peak": 65536 * flop / (inf_ms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+              "train_step_b4096_ms": tr_ms,
+              "train_tflops": 3 * 4096 * flop / (tr_ms * 1e-3) / 1e12}
+    if not fits:
+        net.close()
+        return timing
     rng = np.random.default_rng(7)
     truth = fit_bench.draw_truth(rng)
     y, scale = F.lab_curve(ctx, truth, rng=rng)
@@ -138,11 +149,7 @@ def surrogate_extras(ctx) -> dict:
     rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=150, seed=1)
     rm = F.fit_mc(ctx, y, scale, truth[5], pop=128, gens=40, seed=1)
     net.close()
-    return {"inference_b65536_ms": inf_ms,
-            "inference_tflops": 65536 * flop / (inf_ms * 1e-3) / 1e12,
-            "inference_frac_bf16_peak": 65536 * flop / (inf_ms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
-            "train_step_b4096_ms": tr_ms,
-            "train_tflops": 3 * 4096 * flop / (tr_ms * 1e-3) / 1e12,
+    return {**timing,
             "fit_seconds_surrogate": rs.seconds, "fit_seconds_mc": rm.seconds,
             "fit_speedup": rm.seconds / rs.seconds, "fit_judge_nrmse": rs.judge_nrmse,
             "fit_fallback": rs.fallback,
@@ -162,7 +169,28 @@ def main() -> int:
     ap.add_argument("--sims", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the glow-curve and surrogate figures")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only time the surrogate network (training step, inference batch)")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="only run a plain device copy of this many GiB three times (the "
+                         "reference point for hardware byte counters) and exit")
     args = ap.parse_args()
+    if args.surrogate_only:
+        from mcluminescence_amd._abi import Context
+        ctx = Context(0)
+        print(json.dumps(surrogate_extras(ctx, fits=False)), flush=True)
+        ctx.close()
+        return 0
+    if args.copy_gib > 0:
+        import torch
+        n = int(args.copy_gib * 2 ** 30) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 3}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the Monte Carlo shards clusters with no "
@@ -236,7 +264,8 @@ def main() -> int:
         "curves_per_s": args.steps * n / wall,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12,
                      "peak": PEAK_VALU_LANE_SLOTS / 1e12, "unit": "Tslot/s",
-                     "frac": achieved / PEAK_VALU_LANE_SLOTS, "traffic": None,
+                     "frac": achieved / PEAK_VALU_LANE_SLOTS,
+                     "traffic": TRAFFIC_BYTES_DECAY if args.sims == 4096 else None,
                      "kernel_ms": mean_ms,
                      "hbm_gbps_algorithmic": out_bytes / (mean_ms * 1e-3) / 1e9,
                      "hbm_peak_gbps": PEAK_HBM_GBPS},
