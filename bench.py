@@ -251,7 +251,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "f32+f64",
         "data": "synthetic (Philox-generated geometry, seed 0x4D434C554D000001)",
         "config": {"workload": "iso_decay_500K_4096x400", "clusters": n, "electrons": 400,
                    "holes": 691, "T_K": 500.0, "dt_s": 0.5, "n_steps": 4000, "E_eV": 1.4,
