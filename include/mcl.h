@@ -188,6 +188,30 @@ void mcl_plan_destroy(void* plan);
 int  mcl_sync(void* ctx);
 
 /*
+ * Event-mode Monte Carlo: the reference's loop itself (simulate.py:46-92), one
+ * event per step, for its exact step / lum / electron_ratio rows.  Per step the
+ * total rate K = sum of every live electron's k_e (engine.py:65-77, channel as in
+ * the fixed-dt step) gives dt_rec = -ln(U) / K (the earliest of the electrons'
+ * exponential waits, tl_trap_lab.py:48-51), dt = min(dt_fill, dt_rec, dt_cap) with
+ * dt_cap = dt / T_rate (the run's dt field holds max_dt; no cap at T_rate = 0),
+ * and a recombination picks electron e with probability k_e / K; it and its
+ * nearest hole are removed (Box.remove_pair, engine.py:154-175).
+ *   duration  [n_runs] seconds: the loop runs while t <= duration and stops after
+ *             the step that reaches it (simulate.py:51,91).
+ *   lum       [total_clusters][max_steps] 1 for a recombination step, else 0
+ *   t_axis    [total_clusters][max_steps] time after each step (x_ax), or NULL
+ *   e_ratio   [total_clusters][max_steps] n_e / N_e after each step, or NULL
+ *   steps     [total_clusters] rows used (negative: -1 - rows when max_steps ran
+ *             out); rows after the last are zero, as in the reference's padding.
+ * Returns MCL_EOVERFLOW when a cluster needs more than max_steps steps (the
+ * reference raises IndexError), MCL_EINVAL for D != 0 or a filling event that
+ * would continue the run.  The run's n_steps must be >= 1 and is not used.
+ */
+int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* duration,
+                  uint64_t seed, int32_t max_steps, uint32_t flags, uint8_t* lum,
+                  double* t_axis, double* e_ratio, int32_t* steps, mcl_stats* stats);
+
+/*
  * State of cluster (run_index, sim) as the kernel holds it after max_steps
  * steps (0 = the initial state): coordinates [nm] in float32, which electrons
  * and holes are left, the incrementally maintained nearest live hole per

@@ -134,6 +134,8 @@ _SIGNATURES = {
     "mcl_sync": (C.c_int, [_P]),
     "mcl_cluster_state": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
+    "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
+                                C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
     # surrogate network (include/mcl_surrogate.h)
     "mcl_mlp_create": (C.c_int, [_P, C.POINTER(MlpDesc), C.POINTER(_P)]),
     "mcl_mlp_destroy": (None, [_P]),
@@ -260,6 +262,28 @@ class Context:
                                        flags, _ptr(counts), _ptr(live), C.byref(stats))
         self.check(st, "mcl_run_fixed_dt")
         return counts, live, stats
+
+    def run_event(self, runs: Sequence[RunParams], duration: Sequence[float], max_steps: int,
+                  seed: int = DEFAULT_SEED):
+        """Event-mode Monte Carlo (one event per step; each run's dt field is max_dt).
+        Returns lum [clusters, max_steps] uint8, t [clusters, max_steps] float64,
+        e_ratio [clusters, max_steps] float64, steps [clusters] int32, Stats.  Raises
+        MclError with status MCL_EOVERFLOW when a cluster needs more than max_steps."""
+        arr = run_array(runs)
+        total = sum(int(r.sims) for r in runs)
+        dur = np.ascontiguousarray(np.asarray(duration, dtype=np.float64))
+        if dur.size != len(runs):
+            raise ValueError("one duration per run")
+        lum = np.zeros((total, max_steps), dtype=np.uint8)
+        t = np.zeros((total, max_steps), dtype=np.float64)
+        er = np.zeros((total, max_steps), dtype=np.float64)
+        steps = np.zeros(total, dtype=np.int32)
+        stats = Stats()
+        st = self.lib.mcl_run_event(self.handle, arr, len(runs), _ptr(dur), C.c_uint64(seed),
+                                    int(max_steps), 0, _ptr(lum), _ptr(t), _ptr(er), _ptr(steps),
+                                    C.byref(stats))
+        self.check(st, "mcl_run_event")
+        return lum, t, er, steps, stats
 
     def cluster_state(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
                       max_steps: int = 0) -> dict:

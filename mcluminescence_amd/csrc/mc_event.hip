@@ -1,0 +1,264 @@
+// mc_event.hip — event-mode trap/charge-kinetics Monte Carlo, one wave64 per
+// cluster, one event per step: the reference's loop (simulate.py:46-92 driving
+// TLTrapSim._update_lifetimes / _filling_time, tl_trap_lab.py:48-60, and
+// Box.remove_pair, engine.py:154-175), for the exact step / lum / electron_ratio
+// semantics of its CSV.  Geometry and nearest holes are those of the fixed-dt
+// kernel (mc_common.h).
+//
+// Step i, with t the time so far (float64):
+//   T = (T_start + T_rate t) + 273.15 (simulate.py:53-54); A = s e^{-E_cb/kT} and
+//   B_c = b e^{-E_loc_c/kT} in float64 with the library's exp; per live electron
+//   (index order) k_e = fma(B_ch, w_e, A) in float64, the channel drawn as in the
+//   fixed-dt step (Philox (i >> 2, e, run, sim|CHAN), word i & 3).
+//   K = sum of k_e: each row of 64 consecutive live electrons is summed by an
+//   inclusive scan over offsets 1, 2, 4, ..., 32, and the rows are added in order.
+//   One Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives three words:
+//   E_t = -ln((u0 + 1/2) 2^-32), U_w = u1 2^-32, E_f = -ln((u2 + 1/2) 2^-32).
+//   dt_fill = E_f (1 / lambda), lambda = 1e-20 when D = 0 or n_e = N_e
+//   (tl_trap_lab.py:53-60); dt_rec = E_t / K: the earliest of independent
+//   exponential waits with rates k_e (tl_trap_lab.py:48-51) is exponential with
+//   rate K; with no electron dt_rec = dt_fill, and +inf when every rate is 0
+//   (simulate.py:59).  dt = min(dt_fill, dt_rec, dt_cap), dt_cap = max_dt / T_rate,
+//   or 1e20 without a ramp (simulate.py:45,60).
+//   t += dt.  dt == dt_fill: a filling event (only the terminal one of an emptied
+//   isothermal run exists without dose); else dt == dt_rec: the recombining
+//   electron is the first whose running sum exceeds U_w K, i.e. electron e with
+//   probability k_e / K, the law of the argmin of the waits.  It and its nearest
+//   hole are removed, the live electrons are compacted (index order kept), and
+//   the electrons that had that hole are searched again.
+//   Row i: lum = 1 for a recombination, else 0; t; n_e / N_e.  The loop runs while
+//   t <= duration and stops after a step that reaches duration (simulate.py:51,91).
+// The oracle (oracle/mcl_oracle.c, orc_run_event) replays the same arithmetic in
+// the same order, so every row is bit-exact.
+#include "mc_common.h"
+
+namespace mcl {
+
+// Inclusive prefix sum over the 64 lanes (offsets 1, 2, ..., 32).
+__device__ __forceinline__ double row_scan(double v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double u = __shfl_up(v, o, kWave);
+        if (lane >= o) v = v + u;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double exp1_of(uint32_t u)
+{
+    return -log_own(((double)u + 0.5) * 0x1p-32);
+}
+
+__device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int c,
+                                                  unsigned char* lds_raw)
+{
+    const int lane = (int)threadIdx.x;
+    const int run = uniform(a.c_run[c]);
+    const int sim = uniform(a.c_sim[c]);
+    const RunDesc R = a.runs[run];
+    const int n_e0 = uniform(R.n_e0);
+    const int h_cap = a.h_cap, e_cap = a.e_cap, g_cap = a.g_cap;
+
+    float4* hole = (float4*)lds_raw;
+    float* hole_f = (float*)lds_raw;
+    uint32_t* cstart = (uint32_t*)(hole + h_cap);
+    uint32_t* est = cstart + g_cap;
+    float* ew = (float*)(est + e_cap);
+    double* rtot = (double*)(ew + e_cap);            // row sums (the ethr words)
+    uint32_t* rq = (uint32_t*)(ew + e_cap) + e_cap;
+
+    const uint32_t k0 = a.key0, k1 = a.key1;
+    const uint32_t prun = (uint32_t)R.run_id;
+    const uint32_t psim = (uint32_t)sim << 4;
+    const ElectronGen egen{prun, psim | MCL_TAG_GEOM_E, k0, k1, R.side_nm};
+
+    uint64_t dist_evals = 0;
+    const Grid grid = setup_cluster(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, lane,
+                                    dist_evals);
+    const float alpha_f = (float)R.alpha_nm;
+
+    const bool two_ch = R.two_channel != 0;
+    const bool always2 = R.always_chan2 != 0;
+    const uint64_t ret_thr = R.ret_thr;
+    const uint32_t c3f = psim | MCL_TAG_FIRE, c3c = psim | MCL_TAG_CHAN;
+    const bool b1_is_a = R.b == R.s && R.E_loc_1 == R.E_cb;
+    const bool b2_is_b1 = R.E_loc_2 == R.E_loc_1;
+    const double dt_cap = R.T_rate != 0.0 ? R.dt / R.T_rate : 1e20;
+    const int N_int = (int)R.N_e;
+    const int max_steps = a.max_steps;
+    uint8_t* lum = a.lum + (size_t)c * (size_t)max_steps;
+    double* tax = a.t_axis ? a.t_axis + (size_t)c * (size_t)max_steps : nullptr;
+    double* erat = a.e_ratio ? a.e_ratio + (size_t)c * (size_t)max_steps : nullptr;
+
+    int n_live = n_e0;        // live electrons occupy slots 0 .. n_live - 1
+    int filled = 0;
+    double t = 0.0;
+    int i = 0;
+    int32_t status = 0;
+    uint64_t lsteps = 0, recomb = 0;
+
+    while (t <= R.duration) {
+        if (i >= max_steps) {
+            status = -1 - i;
+            break;
+        }
+        lsteps += (uint64_t)n_live;
+        const double T = (R.T_start_C + R.T_rate * t) + 273.15;
+        const double kT = R.k_b * T;
+        const double A = R.s * exp_own(-(R.E_cb / kT));
+        const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
+        const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
+        const uint32_t blk = (uint32_t)i >> 2;
+        const int word = i & 3;
+        auto rate_of = [&](int p) -> double {
+            if (p >= n_live) return 0.0;
+            const uint32_t st = est[p];
+            if ((st >> 16) == kNoHole) return 0.0;
+            bool ch = always2;
+            if (two_ch)
+                ch = (uint64_t)pick_word(philox10(blk, st & 0xFFFFu, prun, c3c, k0, k1), word) <
+                     ret_thr;
+            return __builtin_fma(ch ? B2 : B1, (double)ew[p], A);
+        };
+        const int rows = (n_live + kWave - 1) / kWave;
+        double K = 0.0;
+        for (int r = 0; r < rows; ++r) {
+            const double v = row_scan(rate_of(r * kWave + lane), lane);
+            const double rt = __shfl(v, kWave - 1, kWave);
+            if (lane == 0) rtot[r] = rt;
+            K = K + rt;
+        }
+        __syncthreads();
+        const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
+        const double lam = (R.D == 0.0 || n_live + filled == N_int)
+                               ? 1e-20
+                               : (R.D / R.D0) * (double)(N_int - (n_live + filled));
+        const double dt_fill = (1.0 / lam) * exp1_of(sb.z);
+        double dt_rec = dt_fill;
+        if (n_live > 0) dt_rec = K > 0.0 ? exp1_of(sb.x) / K : __builtin_inf();
+        const double dt = fmin(fmin(dt_fill, dt_rec), dt_cap);
+        t += dt;
+        uint8_t ev = 0;
+        if (dt == dt_fill) {
+            filled += 1;
+            if (!(t >= R.duration && R.duration != 0.0)) {
+                status = kEventFillUnsupported;   // the new pair would take part
+                break;
+            }
+        } else if (dt == dt_rec) {
+            // the recombining electron: first running sum above U_w K
+            const double target = (double)sb.y * 0x1p-32 * K;
+            int pw = -1;
+            double carry = 0.0;
+            for (int r = 0; r < rows; ++r) {
+                const double c2 = carry + rtot[r];
+                if (c2 > target) {
+                    const double v = row_scan(rate_of(r * kWave + lane), lane);
+                    const uint64_t m = ballot(carry + v > target);
+                    pw = r * kWave + (int)__builtin_ctzll(m);
+                    break;
+                }
+                carry = c2;
+            }
+            if (pw < 0) {
+                // U_w K rounded up to K: the last electron with a rate
+                for (int r = rows - 1; r >= 0 && pw < 0; --r) {
+                    const uint64_t m = ballot(rate_of(r * kWave + lane) > 0.0);
+                    if (m != 0) pw = r * kWave + 63 - (int)__builtin_clzll(m);
+                }
+            }
+            pw = uniform(pw);
+            const uint32_t sw = est[pw];
+            const uint32_t hw = sw >> 16;
+            __syncthreads();
+            if (lane == 0) hole_f[4 * hw] = kDeadHoleX;
+            // compact the other live electrons (index order kept)
+            int base = 0;
+            for (int r = 0; r < rows; ++r) {
+                const int p = r * kWave + lane;
+                const bool keep = p < n_live && p != pw;
+                const uint32_t st = keep ? est[p] : 0u;
+                const float w = keep ? ew[p] : 0.0f;
+                const uint64_t km = ballot(keep);
+                __syncthreads();
+                if (keep) {
+                    const int q = base + lanes_below(km);
+                    est[q] = st;
+                    ew[q] = w;
+                }
+                base += __popcll(km);
+                __syncthreads();
+            }
+            n_live -= 1;
+            recomb += 1;
+            // electrons that had the removed hole search again (engine.py:168-175)
+            int nq = 0;
+            for (int r = 0; r * kWave < n_live; ++r) {
+                const int p = r * kWave + lane;
+                const uint32_t st = p < n_live ? est[p] : kDeadSlot;
+                const bool s = (st >> 16) == hw;
+                const uint64_t m = ballot(s);
+                if (s) rq[nq + lanes_below(m)] = ((uint32_t)p << 16) | (st & 0xFFFFu);
+                nq += __popcll(m);
+            }
+            __syncthreads();
+            if (nq > 0) {
+                search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane,
+                                dist_evals);
+                __syncthreads();
+            }
+            ev = 1;
+        }
+        if (lane == 0) {
+            lum[i] = ev;
+            if (tax) tax[i] = t;
+            if (erat) erat[i] = (double)(n_live + filled) / R.N_e;
+        }
+        ++i;
+        if (R.duration != 0.0 && t >= R.duration) break;
+    }
+    const int used = i;
+    for (int r = used + lane; r < max_steps; r += kWave) {
+        lum[r] = 0;
+        if (tax) tax[r] = 0.0;
+        if (erat) erat[r] = 0.0;
+    }
+    uint64_t evals_total = dist_evals;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)evals_total, off);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(evals_total >> 32), off);
+        evals_total += ((uint64_t)hi << 32) | lo;
+    }
+    if (lane == 0) {
+        a.steps[c] = status != 0 ? status : used;
+        if (a.work != nullptr) {
+            a.work[4 * (size_t)c] = lsteps;
+            a.work[4 * (size_t)c + 1] = evals_total;
+            a.work[4 * (size_t)c + 2] = recomb;
+            a.work[4 * (size_t)c + 3] = (uint64_t)used;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void event_kernel(EventArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if ((int)blockIdx.x < a.n_clusters) run_cluster_event(a, (int)blockIdx.x, lds_raw);
+}
+
+hipError_t prepare_event(int lds_bytes)
+{
+    return hipFuncSetAttribute((const void*)event_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+}
+
+hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream)
+{
+    hipLaunchKernelGGL(event_kernel, dim3((unsigned)a.n_clusters), dim3(kWave),
+                       (unsigned)lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace mcl

@@ -55,219 +55,9 @@
 //   compact   stable ballot/mbcnt compaction, only when it drops a 64-slot row.
 // Counts go out per bin (bin of step n = n * n_bins / n_steps), zero-padded
 // after the last electron recombines, like the reference's padded rows.
-#include "mcl_device.h"
-#include "mcl_internal.h"
-#include "../../include/mcl.h"
+#include "mc_common.h"
 
 namespace mcl {
-
-__device__ __forceinline__ float unit_coord(uint32_t word, double side)
-{
-    return (float)(((double)word * 0x1p-32) * side);
-}
-
-__device__ __forceinline__ int cell_of(float v, float inv_cell, int G)
-{
-    const int c = (int)(v * inv_cell);   // v >= 0
-    return c < G ? c : G - 1;
-}
-
-__device__ __forceinline__ uint64_t hole_key(float d2, uint32_t meta, uint32_t slot)
-{
-    // (d2 bits, original index, slot): the min is the first-index argmin of d2
-    return ((uint64_t)__float_as_uint(d2) << 32) | ((meta & 0xFFFFu) << 16) | slot;
-}
-
-__device__ __forceinline__ float sq_dist(const float4& h, float x, float y, float z)
-{
-    const float dx = h.x - x, dy = h.y - y, dz = h.z - z;
-    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-}
-
-__device__ __forceinline__ uint64_t key_of(const float4& h, float x, float y, float z, int slot)
-{
-    return hole_key(sq_dist(h, x, y, z), __float_as_uint(h.w), (uint32_t)slot);
-}
-
-// Distance from v to the nearest face of cells [lo, hi] that has cells beyond it.
-__device__ __forceinline__ float face_gap(float v, int lo, int hi, int G, float cell)
-{
-    float g = __builtin_inff();
-    if (lo > 0) g = v - (float)lo * cell;
-    if (hi < G - 1) g = fminf(g, (float)(hi + 1) * cell - v);
-    return g;
-}
-
-// Is the block minimum d2 provably below every hole outside the block?
-__device__ __forceinline__ bool block_conclusive(float best_d2, float gap, float cell)
-{
-    const float m = gap - 1e-4f * cell;
-    return m > 0.0f && best_d2 < m * m * (1.0f - 1e-5f);
-}
-
-struct Grid {
-    const float4* hole;
-    const uint32_t* cstart;
-    int G;
-    float cell, inv_cell;
-    int n_h;
-};
-
-struct ElectronGen {
-    uint32_t run, sim_tag, k0, k1;
-    double side_nm;
-    __device__ __forceinline__ void xyz(uint32_t e, float& x, float& y, float& z) const
-    {
-        const u32x4 o = philox10(0u, e, run, sim_tag, k0, k1);
-        x = unit_coord(o.x, side_nm);
-        y = unit_coord(o.y, side_nm);
-        z = unit_coord(o.z, side_nm);
-    }
-};
-
-constexpr int kChunk = 8;   // hole records fetched per LDS round trip in a search
-
-// Lane-private search of the (2 RAD + 1)^3 cell block around (x, y, z).
-template <int RAD>
-__device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, float y, float z,
-                                                      bool& conclusive, uint64_t& evals)
-{
-    constexpr int W = 2 * RAD + 1;
-    const int G = g.G;
-    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
-              cz = cell_of(z, g.inv_cell, G);
-    const int x0 = max(cx - RAD, 0), x1 = min(cx + RAD, G - 1);
-    uint64_t key = ~0ull;
-#pragma unroll
-    for (int k = 0; k < W * W; ++k) {
-        const int zz = cz + k / W - RAD, yy = cy + k % W - RAD;
-        int s = 0, e = 0;
-        if ((unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
-            const int base = (zz * G + yy) * G;
-            s = (int)g.cstart[base + x0];
-            e = (int)g.cstart[base + x1 + 1];
-        }
-        evals += (uint64_t)(e - s);
-        for (int i0 = s; i0 < e; i0 += kChunk) {
-            float4 hv[kChunk];
-#pragma unroll
-            for (int t = 0; t < kChunk; ++t) hv[t] = g.hole[min(i0 + t, e - 1)];
-#pragma unroll
-            for (int t = 0; t < kChunk; ++t) {
-                const uint64_t kk = key_of(hv[t], x, y, z, min(i0 + t, e - 1));
-                key = kk < key ? kk : key;
-            }
-        }
-    }
-    const int y0 = max(cy - RAD, 0), y1 = min(cy + RAD, G - 1);
-    const int z0 = max(cz - RAD, 0), z1 = min(cz + RAD, G - 1);
-    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
-    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
-                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
-    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
-    return key;
-}
-
-// Wave-cooperative search for one electron (x, y, z wave-uniform): the lanes
-// spread over the concatenated slot ranges of the 3x3x3 block, one hole each,
-// then a wave min.  About 150 VALU instructions per electron against ~1000 for
-// a lane-private pass, so it wins while few electrons are queued.
-__device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, float y, float z,
-                                                      int lane, bool& conclusive, uint64_t& evals)
-{
-    const int G = g.G;
-    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
-              cz = cell_of(z, g.inv_cell, G);
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
-    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
-    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
-    const int ny = y1 - y0 + 1;
-    const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
-    int s = 0, len = 0;
-    if (lane < nr) {
-        const int zz = z0 + lane / ny, yy = y0 + lane % ny;
-        const int base = (zz * G + yy) * G;
-        s = (int)g.cstart[base + x0];
-        len = (int)g.cstart[base + x1 + 1] - s;
-    }
-    int incl = len;                                  // inclusive prefix over lanes 0..15
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, 15);
-    const int excl = incl - len;
-    if (lane == 0) evals += (uint64_t)total;         // summed over lanes at the end
-    uint64_t key = ~0ull;
-    for (int l0 = 0; l0 < total; l0 += kWave) {
-        const int l = l0 + lane;
-        int sel = 0;
-        for (int j = 1; j < nr; ++j) sel = (l >= __shfl(excl, j)) ? j : sel;
-        const int slot = __shfl(s, sel) + (l - __shfl(excl, sel));
-        if (l < total) {
-            const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
-            key = k < key ? k : key;
-        }
-    }
-    key = wave_min_u64(key);
-    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
-    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
-                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
-    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
-    return key;
-}
-
-// The whole wave scans every hole for one electron (x, y, z wave-uniform).
-__device__ __forceinline__ uint64_t full_scan_wave(const Grid& g, float x, float y, float z,
-                                                   int lane, uint64_t& evals)
-{
-    if (lane == 0) evals += (uint64_t)g.n_h;   // evaluations are summed over lanes
-    uint64_t key = ~0ull;
-    for (int i = lane; i < g.n_h; i += kWave) {
-        const uint64_t k = key_of(g.hole[i], x, y, z, i);
-        key = k < key ? k : key;
-    }
-    return wave_min_u64(key);
-}
-
-// Nearest live hole for every active lane's electron: 3x3x3 block search per
-// lane, then one wave-wide full scan per lane whose block was inconclusive.  (A
-// lane-private 5x5x5 level in between measured slower: it is divergent, so one
-// open lane makes the wave pay 25 serialized LDS round trips.)
-__device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x, float y, float z,
-                                               int lane, uint64_t& evals)
-{
-    bool ok = true;
-    uint64_t key = ~0ull;
-    if (act) key = block_search_lane<1>(g, x, y, z, ok, evals);
-    uint64_t fb = ballot(act && !ok);
-    while (fb != 0) {
-        const int l = __builtin_ctzll(fb);
-        fb &= fb - 1;
-        const float fx = __shfl(x, l), fy = __shfl(y, l), fz = __shfl(z, l);
-        const uint64_t k = full_scan_wave(g, fx, fy, fz, lane, evals);
-        if (lane == l) key = k;
-    }
-    return key;
-}
-
-__device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_t& slot, float& w)
-{
-    const uint32_t d2bits = (uint32_t)(key >> 32);
-    const bool none = d2bits >= 0x7f800000u;        // no live hole left
-    slot = none ? kNoHole : (uint32_t)(key & 0xFFFFu);
-    w = none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
-}
-
-// Bit r set iff any lane has bit r of its row mask set (rows <= 64), by ballots.
-__device__ __forceinline__ uint64_t rows_any(uint64_t bits, int rows)
-{
-    uint64_t m = 0;
-    for (int r = 0; r < rows; ++r)
-        if (ballot((bits >> r) & 1ull) != 0) m |= 1ull << r;
-    return m;
-}
 
 // Simulate cluster c from its geometry to its last step, all state in lds_raw.
 __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
@@ -280,8 +70,6 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     const int n_e0 = uniform(R.n_e0);
     const int n_h = uniform(R.n_h);
     const int n_steps = uniform(R.n_steps);
-    const int G = uniform(R.grid);
-    const int n_cells = G * G * G;
     const int h_cap = a.h_cap, e_cap = a.e_cap, g_cap = a.g_cap;
 
     float4* hole = (float4*)lds_raw;
@@ -296,77 +84,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     const uint32_t k0 = a.key0, k1 = a.key1;
     const uint32_t prun = (uint32_t)R.run_id;
     const uint32_t psim = (uint32_t)sim << 4;
-    const float cell = R.cell_nm, inv_cell = R.inv_cell;
     const ElectronGen egen{prun, psim | MCL_TAG_GEOM_E, k0, k1, R.side_nm};
 
-    // ---- holes (Box.seed, engine.py:124-129), counting-sorted by grid cell ------
-    for (int i = lane; i <= n_cells; i += kWave) cstart[i] = 0u;
-    __syncthreads();
-    for (int h = lane; h < n_h; h += kWave) {
-        const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
-        const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
-                    z = unit_coord(o.z, R.side_b_nm);
-        const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
-                       cell_of(x, inv_cell, G);
-        atomicAdd(&cstart[cc + 1], 1u);
-    }
-    __syncthreads();
-    {
-        uint32_t carry = 0;
-        for (int base = 0; base <= n_cells; base += kWave) {
-            const int i = base + lane;
-            uint32_t v = i <= n_cells ? cstart[i] : 0u;
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const uint32_t u = (uint32_t)__shfl_up((int)v, off);
-                if (lane >= off) v += u;
-            }
-            v += carry;
-            if (i <= n_cells) cstart[i] = v;
-            carry = (uint32_t)__shfl((int)v, kWave - 1);
-        }
-    }
-    __syncthreads();
-    for (int h = lane; h < n_h; h += kWave) {
-        const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
-        const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
-                    z = unit_coord(o.z, R.side_b_nm);
-        const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
-                       cell_of(x, inv_cell, G);
-        const uint32_t slot = atomicAdd(&cstart[cc], 1u);
-        hole[slot] = make_float4(x, y, z, __uint_as_float(0xFFFF0000u | (uint32_t)h));
-    }
-    __syncthreads();
-    // cstart[c] now holds the end of cell c: shift right by one (top chunk first)
-    for (int base = (n_cells / kWave) * kWave; base >= 0; base -= kWave) {
-        const int i = base + lane;
-        const uint32_t v = (i >= 1 && i <= n_cells) ? cstart[i - 1] : 0u;
-        __syncthreads();
-        if (i <= n_cells) cstart[i] = v;
-        __syncthreads();
-    }
-
-    const Grid grid{hole, cstart, G, cell, inv_cell, n_h};
-    const float alpha_f = (float)R.alpha_nm;
     uint64_t dist_evals = 0;
-
-    // ---- electrons and their initial nearest hole (Box._rebuild, engine.py:113-119)
-    const int rows0 = (n_e0 + kWave - 1) / kWave;
-    for (int r = 0; r < rows0; ++r) {
-        const int e = r * kWave + lane;
-        const bool act = e < n_e0;
-        float x = 0.0f, y = 0.0f, z = 0.0f;
-        if (act) egen.xyz((uint32_t)e, x, y, z);
-        const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
-        if (act) {
-            uint32_t slot;
-            float w;
-            nn_from_key(key, alpha_f, slot, w);
-            est[e] = (slot << 16) | (uint32_t)e;
-            ew[e] = w;
-        }
-    }
-    __syncthreads();
+    const Grid grid = setup_cluster(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, lane,
+                                    dist_evals);
+    const float alpha_f = (float)R.alpha_nm;
 
     // ---- step loop (simulate.py:475-516) -------------------------------------------
     const int nb = a.n_bins > 0 ? a.n_bins : n_steps;   // n_bins == 0: one bin per step
@@ -419,42 +142,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     // Search again for the queued electrons rq[0..nq): one electron at a time
     // with the whole wave while the queue is short, lane-parallel otherwise.
     auto search_queue = [&](int nq) {
-        if (nq <= a.wave_search_max) {
-            float qx = 0.0f, qy = 0.0f, qz = 0.0f;
-            const uint32_t qq = lane < nq ? rq[lane] : 0u;
-            if (lane < nq) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
-            for (int i = 0; i < nq; ++i) {
-                const uint32_t q = (uint32_t)__shfl((int)qq, i);
-                const float x = __shfl(qx, i), y = __shfl(qy, i), z = __shfl(qz, i);
-                bool ok = true;
-                uint64_t key = block_search_wave(grid, x, y, z, lane, ok, dist_evals);
-                if (!ok) key = full_scan_wave(grid, x, y, z, lane, dist_evals);
-                uint32_t slot;
-                float w;
-                nn_from_key(key, alpha_f, slot, w);
-                if (lane == 0) {
-                    est[q >> 16] = (slot << 16) | (q & 0xFFFFu);
-                    ew[q >> 16] = w;
-                }
-            }
-            return;
-        }
-        for (int q0 = 0; q0 < nq; q0 += kWave) {
-            const int i = q0 + lane;
-            const bool act = i < nq;
-            const uint32_t q = act ? rq[i] : 0u;
-            const uint32_t o = q & 0xFFFFu;
-            float x = 0.0f, y = 0.0f, z = 0.0f;
-            if (act) egen.xyz(o, x, y, z);
-            const uint64_t key = nn_resolve(grid, act, x, y, z, lane, dist_evals);
-            if (act) {
-                uint32_t slot;
-                float w;
-                nn_from_key(key, alpha_f, slot, w);
-                est[q >> 16] = (slot << 16) | o;
-                ew[q >> 16] = w;
-            }
-        }
+        search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane, dist_evals);
     };
 
     // Step scalars of step m: start-of-step temperature, Boltzmann factors in

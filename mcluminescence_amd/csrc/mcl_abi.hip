@@ -135,6 +135,11 @@ mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_i
     d.grid = mcl::grid_dim(g.n_h, holes_per_cell);
     d.cell_nm = (float)(d.side_b_nm / d.grid);
     d.inv_cell = (float)(d.grid / d.side_b_nm);
+    d.T_start_C = p.T_start;
+    d.N_e = p.N_e;
+    d.D = p.D;
+    d.D0 = p.D0;
+    d.duration = 0.0;
     return d;
 }
 
@@ -629,6 +634,166 @@ int mcl_sync(void* ctx)
     Ctx* c = (Ctx*)ctx;
     if (!c) return MCL_EINVAL;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MCL_OK;
+}
+
+int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* duration,
+                  uint64_t seed, int32_t max_steps, uint32_t flags, uint8_t* lum, double* t_axis,
+                  double* e_ratio, int32_t* steps, mcl_stats* stats)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!runs || !duration || !lum || !steps)
+        return fail(c, MCL_EINVAL, "runs, duration, lum and steps must not be null");
+    if (n_runs <= 0) return fail(c, MCL_EINVAL, "n_runs must be > 0");
+    if (max_steps <= 0) return fail(c, MCL_EINVAL, "max_steps must be > 0");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    std::vector<mcl::RunDesc> desc((size_t)n_runs);
+    int64_t total = 0;
+    int max_e = 0, max_h = 0, max_cells = 1;
+    for (int r = 0; r < n_runs; ++r) {
+        mcl_geometry g;
+        const int st = validate_run(c, runs[r], r, 0, g);
+        if (st) return st;
+        const double dur = duration[r];
+        if (!std::isfinite(dur) || dur < 0.0)
+            return fail(c, MCL_EINVAL, "run %d: duration must be finite and >= 0 (got %g)", r, dur);
+        if (!((runs[r].T_start + 273.15) + runs[r].T_rate * dur > 0.0))
+            return fail(c, MCL_EINVAL, "run %d: temperature must stay > 0 K over the duration", r);
+        if (!(runs[r].N_e > 0.0)) return fail(c, MCL_EINVAL, "run %d: N_e must be > 0", r);
+        desc[(size_t)r] = make_desc(runs[r], g, r, c->holes_per_cell);
+        desc[(size_t)r].duration = dur;
+        total += runs[r].sims;
+        if (runs[r].sims > 0) {
+            max_e = std::max(max_e, g.n_e0);
+            max_h = std::max(max_h, g.n_h);
+            const int G = desc[(size_t)r].grid;
+            max_cells = std::max(max_cells, G * G * G);
+        }
+    }
+    if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
+    if (total > 0x7FFFFFFF) return fail(c, MCL_EINVAL, "too many clusters");
+    const int h_cap = ((max_h + 3) / 4) * 4 + 4;
+    const int e_cap = std::max(64, ((max_e + 63) / 64) * 64);
+    const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
+    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
+    if (lds > c->max_lds)
+        return fail(c, MCL_ERANGE, "cluster of %d electrons / %d holes needs %d B of LDS; a "
+                    "workgroup has %d B", max_e, max_h, lds, c->max_lds);
+    std::vector<int32_t> crun((size_t)total), csim((size_t)total);
+    {
+        int64_t k = 0;
+        for (int r = 0; r < n_runs; ++r)
+            for (int s2 = 0; s2 < runs[r].sims; ++s2, ++k) {
+                crun[(size_t)k] = r;
+                csim[(size_t)k] = s2;
+            }
+    }
+    const bool dev = (flags & MCL_F_DEVICE_PTRS) != 0;
+    const size_t rowsz = (size_t)total * (size_t)max_steps;
+    std::vector<void*> tmp;
+    auto dmalloc = [&](void** ptr, size_t bytes) {
+        const hipError_t e = hipMalloc(ptr, bytes);
+        if (e == hipSuccess) tmp.push_back(*ptr);
+        return e;
+    };
+    auto release = [&]() {
+        for (void* q : tmp) (void)hipFree(q);
+    };
+    mcl::RunDesc* d_runs = nullptr;
+    int32_t *d_crun = nullptr, *d_csim = nullptr, *d_steps = nullptr;
+    uint64_t* d_work = nullptr;
+    uint8_t* d_lum = dev ? lum : nullptr;
+    double* d_t = dev ? t_axis : nullptr;
+    double* d_er = dev ? e_ratio : nullptr;
+    hipError_t e = dmalloc((void**)&d_runs, sizeof(mcl::RunDesc) * desc.size());
+    if (e == hipSuccess) e = dmalloc((void**)&d_crun, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_csim, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_steps, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_work, sizeof(uint64_t) * 4 * (size_t)total);
+    if (e == hipSuccess && !dev) e = dmalloc((void**)&d_lum, rowsz);
+    if (e == hipSuccess && !dev && t_axis) e = dmalloc((void**)&d_t, sizeof(double) * rowsz);
+    if (e == hipSuccess && !dev && e_ratio) e = dmalloc((void**)&d_er, sizeof(double) * rowsz);
+    if (e != hipSuccess) {
+        release();
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "allocation: %s",
+                    hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(d_runs, desc.data(), sizeof(mcl::RunDesc) * desc.size(),
+                       hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_crun, crun.data(), sizeof(int32_t) * (size_t)total,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_csim, csim.data(), sizeof(int32_t) * (size_t)total,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = mcl::prepare_event(lds);
+    mcl::EventArgs a{};
+    a.runs = d_runs;
+    a.c_run = d_crun;
+    a.c_sim = d_csim;
+    a.n_clusters = (int32_t)total;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.max_steps = max_steps;
+    a.lum = d_lum;
+    a.t_axis = d_t;
+    a.e_ratio = d_er;
+    a.steps = d_steps;
+    a.work = d_work;
+    a.h_cap = h_cap;
+    a.e_cap = e_cap;
+    a.g_cap = g_cap;
+    a.wave_search_max = (int32_t)c->wave_search_max;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    if (e == hipSuccess) e = hipEventRecord(ev0, c->stream);
+    if (e == hipSuccess) e = mcl::launch_event(a, lds, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev1, c->stream);
+    std::vector<int32_t> hs((size_t)total);
+    std::vector<uint64_t> hw(4 * (size_t)total);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hs.data(), d_steps, sizeof(int32_t) * (size_t)total,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hw.data(), d_work, sizeof(uint64_t) * 4 * (size_t)total,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !dev) e = hipMemcpyAsync(lum, d_lum, rowsz, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !dev && t_axis)
+        e = hipMemcpyAsync(t_axis, d_t, sizeof(double) * rowsz, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !dev && e_ratio)
+        e = hipMemcpyAsync(e_ratio, d_er, sizeof(double) * rowsz, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ev0, ev1);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    release();
+    if (e != hipSuccess) return fail(c, MCL_EDEVICE, "event kernel or copy failed: %s",
+                                     hipGetErrorString(e));
+    std::memcpy(steps, hs.data(), sizeof(int32_t) * (size_t)total);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->kernel_ms = ms;
+        stats->clusters = (uint64_t)total;
+        for (int64_t k = 0; k < total; ++k) {
+            stats->live_electron_steps += hw[4 * (size_t)k];
+            stats->recombinations += hw[4 * (size_t)k + 2];
+            stats->cluster_steps += hw[4 * (size_t)k + 3];
+        }
+    }
+    for (int64_t k = 0; k < total; ++k) {
+        if (hs[(size_t)k] == mcl::kEventFillUnsupported)
+            return fail(c, MCL_EINVAL, "cluster %lld: a filling event before the end of the run "
+                        "(dose filling is not part of the event kernel)", (long long)k);
+    }
+    for (int64_t k = 0; k < total; ++k) {
+        if (hs[(size_t)k] < 0)
+            return fail(c, MCL_EOVERFLOW, "cluster %lld (run %d, sim %d) needs more than "
+                        "max_steps=%d steps", (long long)k, crun[(size_t)k], csim[(size_t)k],
+                        max_steps);
+    }
     return MCL_OK;
 }
 

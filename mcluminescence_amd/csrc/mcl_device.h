@@ -140,6 +140,42 @@ __device__ __forceinline__ double exp_own(double x)
 
 // Fire threshold for x = k dt: p = 1 - e^{-x}; an electron with draw u fires
 // iff always || u < thr, thr = (uint32)(p * 2^32).
+// float64 natural log for normal x > 0: x = 2^k m with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1), by the odd series 2 s (1 + s^2/3 + ...
+// + s^22/23) (|s| <= 0.1716), then k ln 2 with the two-part ln 2 of exp_own.
+__device__ __forceinline__ double log_own(double x)
+{
+    if (x != x || x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -__builtin_inf();
+    if (x == __builtin_inf()) return x;
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    int k = (int)((b >> 52) & 0x7ffu) - 1023;
+    b = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m = __longlong_as_double((long long)b);
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m *= 0.5;
+        k += 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double q = s * s;
+    double P = 1.0 / 23.0;
+    P = __builtin_fma(P, q, 1.0 / 21.0);
+    P = __builtin_fma(P, q, 1.0 / 19.0);
+    P = __builtin_fma(P, q, 1.0 / 17.0);
+    P = __builtin_fma(P, q, 1.0 / 15.0);
+    P = __builtin_fma(P, q, 1.0 / 13.0);
+    P = __builtin_fma(P, q, 1.0 / 11.0);
+    P = __builtin_fma(P, q, 1.0 / 9.0);
+    P = __builtin_fma(P, q, 1.0 / 7.0);
+    P = __builtin_fma(P, q, 1.0 / 5.0);
+    P = __builtin_fma(P, q, 1.0 / 3.0);
+    const double t = 2.0 * s;
+    const double lnm = __builtin_fma(t, q * P, t);
+    const double kd = (double)k;
+    return __builtin_fma(kd, kDLn2Hi, __builtin_fma(kd, kDLn2Lo, lnm));
+}
+
 struct FireThr { uint32_t thr; bool always; };
 
 __device__ __forceinline__ FireThr fire_threshold(float x)

@@ -25,6 +25,11 @@ struct RunDesc {
     float cell_nm;         // cell edge = side_b_nm / G
     float inv_cell;        // G / side_b_nm
     int32_t pad_;
+    // event mode (the reference loop, simulate.py:46-92)
+    double T_start_C;      // T_start [degC]: T = T_start + T_rate t + 273.15
+    double N_e;            // trap count: electron_ratio = n_e / N_e
+    double D, D0;          // dose rate [Gy/s] and D0 of the filling rate
+    double duration;       // [s] the loop runs while t <= duration
 };
 
 // Optional dump of one cluster's kernel state (tests / inspection).
@@ -62,6 +67,25 @@ struct LaunchArgs {
     StateDump dump;
 };
 
+// Event-mode launch: one cluster per workgroup, one event per step.
+struct EventArgs {
+    const RunDesc* runs;
+    const int32_t* c_run;   // [n_clusters]
+    const int32_t* c_sim;   // [n_clusters]
+    int32_t n_clusters;
+    uint32_t key0, key1;
+    int32_t max_steps;      // rows per cluster
+    uint8_t* lum;           // [n_clusters][max_steps]: 1 for a recombination step
+    double* t_axis;         // [n_clusters][max_steps] time after the step, or null
+    double* e_ratio;        // [n_clusters][max_steps] n_e / N_e after the step, or null
+    int32_t* steps;         // [n_clusters] rows used; -1 - rows when max_steps ran out,
+                            // kEventFillUnsupported when a filling event would continue
+    uint64_t* work;         // [n_clusters][4] or null
+    int32_t h_cap, e_cap, g_cap;
+    int32_t wave_search_max;
+};
+constexpr int32_t kEventFillUnsupported = -0x7fffffff;
+
 constexpr int kWaveHost = 64;   // wavefront width of gfx950
 
 // LDS bytes the fixed-dt kernel needs for the given capacities.
@@ -83,5 +107,7 @@ hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, hipStre
 hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream);
 hipError_t prepare_fixed_dt(int lds_bytes);
 hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu);
+hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
+hipError_t prepare_event(int lds_bytes);
 
 }  // namespace mcl

@@ -10,18 +10,25 @@
   * it writes ``results/simulations/exp_{tag}.csv`` with columns
     ``run,sim,step,lum,electron_ratio``, run-major then sim then step.
 
-What runs underneath is one GPU launch for every (run, sim) cluster: the
-fixed-dt step of the kernels (several recombinations per step).  So ``Lum`` is
-the number of recombinations in each step (0/1 in the reference's one-event
-steps), ``x_ax`` the end time of each step, and ``electron_ratio`` the trapped
-fraction after it.  The step is the reference's temperature-step cap
-``max_dt / T_rate`` for ramps (simulate.py:468-469) and ``dt`` (or ``max_dt``)
-for isothermal runs.  A schedule that needs more than ``steps`` steps raises
-IndexError, as the reference does when its step buffer overflows.
+What runs underneath is one GPU launch for every (run, sim) cluster, in one of
+two modes (config key ``mode``, default ``fixed_dt``):
+
+  fixed_dt  the fixed-dt step of the kernels (several recombinations per step):
+            ``Lum`` is the number of recombinations in each step, ``x_ax`` the end
+            time of each step, ``electron_ratio`` the trapped fraction after it.  The
+            step is the reference's temperature-step cap ``max_dt / T_rate`` for
+            ramps (simulate.py:468-469) and ``dt`` (or ``max_dt``) for isothermal runs.
+  event     the reference's own loop, one event per step (``Lum`` in {0, 1}, the
+            step cap ``max_dt / T_rate``, ``x_ax`` the event times), on the event
+            kernel.
+
+A run that needs more than ``steps`` steps raises IndexError, as the reference
+does when its step buffer overflows.
 
 Command line, like ``python src/class/simulate.py [overrides]``:
 
     python -m mcluminescence_amd.simulate +tag=TLwithDose exp_type_fp.e_ratio_start=0.05
+    python -m mcluminescence_amd.simulate +tag=TL12event +mode=event
 """
 from __future__ import annotations
 
@@ -33,7 +40,7 @@ from typing import Dict, Optional, Sequence, Tuple
 import numpy as np
 
 from . import paths
-from ._abi import DEFAULT_SEED, Context
+from ._abi import DEFAULT_SEED, MCL_EOVERFLOW, Context, MclError
 from .config import Config, compose
 from .engine import initialize_runs, run_params, schedule
 
@@ -76,6 +83,31 @@ def run_sweep(configs: Dict[int, Config], steps: int, sims: int, *, ctx: Optiona
     return x_ax, lum, e_ratio, scheds
 
 
+def run_sweep_event(configs: Dict[int, Config], steps: int, sims: int, *,
+                    ctx: Optional[Context] = None, seed: int = DEFAULT_SEED):
+    """The reference's event loop for every run x sim (simulate.py:460-516), one
+    kernel launch; returns x_ax, Lum, electron_ratio ([steps, sims, n_runs])."""
+    ctx = ctx or default_context()
+    n_runs = len(configs)
+    params, durations = [], []
+    for r in range(n_runs):
+        mc = configs[r].exp_type_fp
+        max_dt = float(mc.get("max_dt", 0.5))                 # simulate.py:468
+        params.append(run_params(configs[r], sims=sims, n_steps=1, dt=max_dt))
+        durations.append(float(mc.get("duration", 0.0)))
+    try:
+        lum, t, er, used, _ = ctx.run_event(params, durations, max_steps=steps, seed=seed)
+    except MclError as e:
+        if e.code == MCL_EOVERFLOW:
+            raise IndexError(f"a run needs more than exp_type_fp.steps = {steps} steps") from e
+        raise
+    shape = (n_runs, sims, steps)
+    x_ax = t.reshape(shape).transpose(2, 1, 0).copy()
+    lm = lum.reshape(shape).transpose(2, 1, 0).astype(np.float64)
+    e_ratio = er.reshape(shape).transpose(2, 1, 0).copy()
+    return x_ax, lm, e_ratio, used.reshape(n_runs, sims)
+
+
 def results_frame(lum: np.ndarray, e_ratio: np.ndarray):
     """The reference's CSV layout (simulate.py:519-532): run-major, then sim, then step."""
     import pandas as pd
@@ -103,7 +135,13 @@ def simulate(cfg: Config, *, ctx: Optional[Context] = None, seed: int = DEFAULT_
     mc0 = configs[0].exp_type_fp
     steps = int(mc0.steps)
     sims = int(mc0.sims)
-    x_ax, lum, e_ratio, _ = run_sweep(configs, steps, sims, ctx=ctx, seed=seed)
+    mode = str(cfg.get("mode", "fixed_dt"))
+    if mode == "event":
+        x_ax, lum, e_ratio, _ = run_sweep_event(configs, steps, sims, ctx=ctx, seed=seed)
+    elif mode == "fixed_dt":
+        x_ax, lum, e_ratio, _ = run_sweep(configs, steps, sims, ctx=ctx, seed=seed)
+    else:
+        raise ValueError(f"mode must be 'fixed_dt' or 'event' (got {mode!r})")
     if write_csv:
         tag = cfg.get("tag", "")
         out_dir = Path(results_dir) if results_dir is not None else paths.simulations_dir()

@@ -166,6 +166,41 @@ double orc_exp(double x)
     return e * d_pow2i((int)kd);
 }
 
+/* float64 natural log for normal x > 0: x = 2^k m with m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(s), s = (m - 1) / (m + 1), by the odd series 2 s (1 + s^2/3 + ...
+ * + s^22/23) (|s| <= 0.1716), then k ln 2 with the same two-part ln 2 as exp. */
+double orc_log(double x)
+{
+    if (x != x || x < 0.0) return NAN;
+    if (x == 0.0) return -INFINITY;
+    if (x == INFINITY) return x;
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    int k = (int)((b >> 52) & 0x7ff) - 1023;
+    b = (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull;
+    double m;
+    memcpy(&m, &b, sizeof m);
+    if (m > 0x1.6a09e667f3bcdp+0) { m *= 0.5; k += 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double q = s * s;
+    double P = 1.0 / 23.0;
+    P = fma(P, q, 1.0 / 21.0);
+    P = fma(P, q, 1.0 / 19.0);
+    P = fma(P, q, 1.0 / 17.0);
+    P = fma(P, q, 1.0 / 15.0);
+    P = fma(P, q, 1.0 / 13.0);
+    P = fma(P, q, 1.0 / 11.0);
+    P = fma(P, q, 1.0 / 9.0);
+    P = fma(P, q, 1.0 / 7.0);
+    P = fma(P, q, 1.0 / 5.0);
+    P = fma(P, q, 1.0 / 3.0);
+    double t = 2.0 * s;
+    double lnm = fma(t, q * P, t);
+    double kd = (double)k;
+    return fma(kd, D_LN2_HI, fma(kd, D_LN2_LO, lnm));
+}
+
 /* -------------------------------------------------------------- geometry -- */
 int orc_run_geometry(const mcl_run_params* p, mcl_geometry* g)
 {
@@ -592,3 +627,174 @@ uint64_t orc_thr_monotone_scan(float lo, float hi, uint64_t* worst)
 
 /* The float32 fire threshold T(x) in [0, 2^32] (2^32: always fires). */
 uint64_t orc_fire_threshold(float x) { return thr_f32(x); }
+
+/* ------------------------------------------------------------ event mode -- */
+/* Replay of the event kernel (mcluminescence_amd/csrc/mc_event.hip), the
+ * reference's loop simulate.py:46-92 with one event per step:
+ *   T = (T_start + T_rate t) + 273.15; A, B_c in float64 (own exp);
+ *   k_e = fma(B_ch, w_e, A) per live electron in index order;
+ *   K = rows of 64 electrons, each an inclusive scan over offsets 1..32, summed
+ *   in order; one Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives
+ *   E_t = -ln((u0 + 1/2) 2^-32), U_w = u1 2^-32, E_f = -ln((u2 + 1/2) 2^-32);
+ *   dt_fill = (1 / lambda) E_f, dt_rec = E_t / K (dt_fill without electrons, +inf
+ *   when K = 0), dt = min(dt_fill, dt_rec, dt_cap); a recombination takes the
+ *   first electron whose running sum exceeds U_w K. */
+static void scan64(double* v)
+{
+    double old[64];
+    for (int o = 1; o < 64; o <<= 1) {
+        memcpy(old, v, sizeof old);
+        for (int l = o; l < 64; ++l) v[l] = old[l] + old[l - o];
+    }
+}
+
+typedef struct {
+    const cluster_t* c;
+    const uint32_t* key;
+    uint32_t run, c3c;
+    int i, two_ch, always2;
+    uint64_t ret_thr;
+    double A, B1, B2;
+} ev_rates_t;
+
+static double ev_rate(const ev_rates_t* R, int slot)
+{
+    if (slot >= R->c->n_live) return 0.0;
+    const int e = R->c->live[slot];
+    if (R->c->nn[e] == MCL_NO_HOLE) return 0.0;
+    int ch = R->always2;
+    if (R->two_ch)
+        ch = (uint64_t)philox_word((uint32_t)R->i >> 2, (uint32_t)e, R->run, R->c3c, R->key,
+                                   R->i & 3) < R->ret_thr;
+    return fma(ch ? R->B2 : R->B1, (double)R->c->w[e], R->A);
+}
+
+static void ev_row(const ev_rates_t* R, int r, double* v)
+{
+    for (int l = 0; l < 64; ++l) v[l] = ev_rate(R, r * 64 + l);
+    scan64(v);
+}
+
+static int32_t run_cluster_event(const mcl_run_params* p, int run, int sim, const uint32_t key[2],
+                                 double duration, int max_steps, uint8_t* lum, double* tax,
+                                 double* erat, uint64_t* live_steps)
+{
+    cluster_t c;
+    cluster_init(&c, p, run, sim, key, 0);
+    const double alpha_nm = p->alpha * 1e-9;
+    const float alpha_f = (float)alpha_nm;
+    const int two_ch = (p->E_loc_1 != p->E_loc_2) && (p->Retrap > 0.0) && (p->Retrap < 1.0);
+    ev_rates_t R = {&c, key, (uint32_t)run, ((uint32_t)sim << 4) | MCL_TAG_CHAN, 0, two_ch,
+                    p->Retrap >= 1.0, two_ch ? (uint64_t)(p->Retrap * 0x1p32) : 0u, 0, 0, 0};
+    const int b1_is_a = p->b == p->s && p->E_loc_1 == p->E_cb;
+    const int b2_is_b1 = p->E_loc_2 == p->E_loc_1;
+    const double dt_cap = p->T_rate != 0.0 ? p->dt / p->T_rate : 1e20;
+    const int N_int = (int)p->N_e;
+    const uint32_t c3f = ((uint32_t)sim << 4) | MCL_TAG_FIRE;
+    double* rtot = xcalloc((size_t)(c.n_e0 / 64 + 2), sizeof(double));
+    double v[64];
+    int filled = 0, i = 0;
+    int32_t status = 0;
+    double t = 0.0;
+    uint64_t ls = 0;
+    memset(lum, 0, (size_t)max_steps);
+    if (tax) memset(tax, 0, sizeof(double) * (size_t)max_steps);
+    if (erat) memset(erat, 0, sizeof(double) * (size_t)max_steps);
+    while (t <= duration) {
+        if (i >= max_steps) { status = -1 - i; break; }
+        ls += (uint64_t)c.n_live;
+        const double T = (p->T_start + p->T_rate * t) + 273.15;
+        const double kT = p->k_b * T;
+        R.A = p->s * orc_exp(-(p->E_cb / kT));
+        R.B1 = b1_is_a ? R.A : p->b * orc_exp(-(p->E_loc_1 / kT));
+        R.B2 = b2_is_b1 ? R.B1 : p->b * orc_exp(-(p->E_loc_2 / kT));
+        R.i = i;
+        const int rows = (c.n_live + 63) / 64;
+        double K = 0.0;
+        for (int r = 0; r < rows; ++r) {
+            ev_row(&R, r, v);
+            rtot[r] = v[63];
+            K = K + v[63];
+        }
+        uint32_t sb[4];
+        {
+            uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFFu, (uint32_t)run, c3f};
+            orc_philox4x32_10(ctr, key, sb);
+        }
+        const double lam = (p->D == 0.0 || c.n_live + filled == N_int)
+                               ? 1e-20 : (p->D / p->D0) * (double)(N_int - (c.n_live + filled));
+        const double dt_fill = (1.0 / lam) * -orc_log(((double)sb[2] + 0.5) * 0x1p-32);
+        double dt_rec = dt_fill;
+        if (c.n_live > 0) dt_rec = K > 0.0 ? -orc_log(((double)sb[0] + 0.5) * 0x1p-32) / K : INFINITY;
+        const double dt = fmin(fmin(dt_fill, dt_rec), dt_cap);
+        t += dt;
+        uint8_t ev = 0;
+        if (dt == dt_fill) {
+            filled += 1;
+            if (!(t >= duration && duration != 0.0)) { status = INT32_MIN + 1; break; }
+        } else if (dt == dt_rec) {
+            const double target = (double)sb[1] * 0x1p-32 * K;
+            int pw = -1;
+            double carry = 0.0;
+            for (int r = 0; r < rows && pw < 0; ++r) {
+                const double c2 = carry + rtot[r];
+                if (c2 > target) {
+                    ev_row(&R, r, v);
+                    for (int l = 0; l < 64; ++l)
+                        if (carry + v[l] > target) { pw = r * 64 + l; break; }
+                    break;
+                }
+                carry = c2;
+            }
+            if (pw < 0)
+                for (int s2 = c.n_live - 1; s2 >= 0 && pw < 0; --s2)
+                    if (ev_rate(&R, s2) > 0.0) pw = s2;
+            const int ew_ = c.live[pw];
+            const uint32_t hw = c.nn[ew_];
+            c.h_alive[hw] = 0;
+            memmove(c.live + pw, c.live + pw + 1, sizeof(int32_t) * (size_t)(c.n_live - pw - 1));
+            c.n_live -= 1;
+            for (int s2 = 0; s2 < c.n_live; ++s2) {
+                const int e = c.live[s2];
+                if (c.nn[e] == hw) refresh_nn(&c, e, alpha_f, alpha_nm, 0);
+            }
+            ev = 1;
+        }
+        lum[i] = ev;
+        if (tax) tax[i] = t;
+        if (erat) erat[i] = (double)(c.n_live + filled) / p->N_e;
+        ++i;
+        if (duration != 0.0 && t >= duration) break;
+    }
+    if (live_steps) *live_steps = ls;
+    free(rtot);
+    cluster_free(&c);
+    return status != 0 ? status : i;
+}
+
+int orc_run_event(const mcl_run_params* runs, int32_t n_runs, const double* duration,
+                  uint64_t seed, int32_t max_steps, int32_t n_threads, uint8_t* lum,
+                  double* t_axis, double* e_ratio, int32_t* steps)
+{
+    if (!runs || n_runs <= 0 || !lum || !steps || !duration || max_steps <= 0) return MCL_EINVAL;
+    int64_t total = 0;
+    for (int r = 0; r < n_runs; ++r) total += runs[r].sims;
+    int32_t* c_run = xcalloc((size_t)total, sizeof(int32_t));
+    int32_t* c_sim = xcalloc((size_t)total, sizeof(int32_t));
+    int64_t k = 0;
+    for (int r = 0; r < n_runs; ++r)
+        for (int s = 0; s < runs[r].sims; ++s) { c_run[k] = r; c_sim[k] = s; ++k; }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t i = 0; i < total; ++i) {
+        const size_t o = (size_t)i * (size_t)max_steps;
+        steps[i] = run_cluster_event(&runs[c_run[i]], c_run[i], c_sim[i], key, duration[c_run[i]],
+                                     max_steps, lum + o, t_axis ? t_axis + o : NULL,
+                                     e_ratio ? e_ratio + o : NULL, NULL);
+    }
+    free(c_run); free(c_sim);
+    return 0;
+}

@@ -69,6 +69,36 @@ def expf(x: float) -> np.float32:
     return np.float32(lib().orc_expf(float(x)))
 
 
+def run_event(runs: Sequence[RunParams], duration, max_steps: int, seed: int = DEFAULT_SEED,
+              threads: int = 0):
+    """Replay of the event kernel: (lum, t, e_ratio, steps) like Context.run_event."""
+    L = lib()
+    L.orc_run_event.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_void_p, C.c_uint64,
+                                C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]
+    L.orc_run_event.restype = C.c_int
+    arr = run_array(runs)
+    total = sum(int(r.sims) for r in runs)
+    dur = np.ascontiguousarray(np.asarray(duration, dtype=np.float64))
+    lum = np.zeros((total, max_steps), dtype=np.uint8)
+    t = np.zeros((total, max_steps), dtype=np.float64)
+    er = np.zeros((total, max_steps), dtype=np.float64)
+    steps = np.zeros(total, dtype=np.int32)
+    st = L.orc_run_event(arr, len(runs), dur.ctypes.data, C.c_uint64(seed), int(max_steps),
+                         int(threads), lum.ctypes.data, t.ctypes.data, er.ctypes.data,
+                         steps.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_run_event failed: {st}")
+    return lum, t, er, steps
+
+
+def log64(x: float) -> float:
+    L = lib()
+    L.orc_log.argtypes = [C.c_double]
+    L.orc_log.restype = C.c_double
+    return float(L.orc_log(float(x)))
+
+
 def fire_threshold(x: float) -> int:
     """T(x) = (uint32)((1 - e^{-x}) 2^32), or 2^32 when p rounds to 1."""
     L = lib()

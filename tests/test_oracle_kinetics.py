@@ -165,3 +165,51 @@ def test_binned_counts_are_sums_of_step_counts():
     for j in range(256):
         want[:, j] = per_step[:, b == j].sum(axis=1)
     np.testing.assert_array_equal(binned, want)
+
+
+def _event_bins(name, sims=1024):
+    fx = json.loads((Path(__file__).parent / "golden" / "ensemble_event_mode.json").read_text())
+    f = fx[name]
+    base = RunParams.make(**f["run"])
+    dt = float(base.dt)
+    p = RunParams.make(**{**base.as_dict(), "sims": sims, "n_steps": 1})
+    p.dt = dt * (base.T_rate if base.T_rate else 1.0)               # max_dt
+    n0 = int(base.N_e * base.e_ratio_start)
+    lum, t, _, steps = orc.run_event([p], [base.n_steps * dt], n0 + int(base.n_steps) + 64,
+                                     threads=8)
+    assert np.all(steps > 0)
+    per, nb = f["steps_per_bin"], f["n_bins"]
+    coarse = np.zeros((sims, nb))
+    for c in range(sims):
+        idx = np.floor(t[c][lum[c] == 1] / dt).astype(np.int64) // per
+        idx = idx[(idx >= 0) & (idx < nb)]
+        np.add.at(coarse[c], idx, 1.0)
+    return f, coarse
+
+
+@pytest.mark.parametrize("name", ["iso_500K_400", "tl_glow_500", "tl12_tunnelling_400"])
+def test_event_replay_matches_event_mode_restatement(name):
+    # The event spec (total-rate wait, winner by k_e / K) against the numpy
+    # restatement of the reference loop (per-electron exponential waits, argmin),
+    # per bin within 3 standard errors (Poisson floor for rare bins).
+    f, coarse = _event_bins(name)
+    S = coarse.shape[0]
+    mg, sg = coarse.mean(axis=0), coarse.std(axis=0, ddof=1)
+    mo, so = np.array(f["mean"]), np.array(f["std"])
+    pooled = (mo * f["sims"] + mg * S) / (f["sims"] + S)
+    floor = np.minimum(pooled, 1.0)
+    se = np.sqrt(np.maximum(so ** 2, floor) / f["sims"] + np.maximum(sg ** 2, floor) / S)
+    z = (mg[se > 0] - mo[se > 0]) / se[se > 0]
+    assert (np.abs(z) > 3).sum() <= 1 and np.abs(z).max() < 4.5, np.abs(z).max()
+
+
+def test_event_replay_terminal_rows_follow_the_reference_loop():
+    # an emptied isothermal run ends with one step of dt = min(dt_fill, 1e20)
+    # (simulate.py:57-60 with dose 0): a fill row (ratio 1/N_e) or a no-event row
+    iso = W.iso_decay_4096x400(sims=64)[0]
+    iso.n_steps = 1
+    lum, t, er, steps = orc.run_event([iso], [2000.0], 600, threads=8)
+    assert np.all(steps == 401)
+    assert np.all(lum[:, :400] == 1) and np.all(lum[:, 400] == 0)
+    assert np.all(t[:, 400] > 1e10)                 # 1e20 x Exp(1) or the 1e20 cap
+    assert set(np.unique(er[:, 400])) <= {0.0, 1.0 / 400}
