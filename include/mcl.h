@@ -212,6 +212,28 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
                   double* t_axis, double* e_ratio, int32_t* steps, mcl_stats* stats);
 
 /*
+ * Lab experiments with dose filling: the loops of TLTrapSim.TL_lab and ISO_lab
+ * (tl_trap_lab.py:65-179).  Each run is one experiment: T_start [degC] and T_rate
+ * [K/s] give T = (T_start + 273.15) + T_rate t, D is the dose rate and
+ * e_ratio_start the initial fill; dt and n_steps are not used.  Events as in
+ * mcl_run_event without a step cap, plus filling at lambda = (D/D0)(N_e - n_e)
+ * (tl_trap_lab.py:53-60): one electron-hole pair per fill (Box.add_electron,
+ * engine.py:133-152).  With exact_fill = 0 the new electron's nearest hole is
+ * searched among the older holes and no other electron looks at the new hole,
+ * the reference's update; exact_fill = 1 updates every nearest hole exactly.
+ * The trapped fraction n_e / N_e after the first step that reaches each
+ * observation time is written to ratio.
+ *   obs_times  all runs' observation times concatenated, ascending within a run
+ *   obs_count  [n_runs] observations per run (>= 1)
+ *   ratio      [total_clusters][max over runs of obs_count]
+ *   status     [total_clusters] 0, -1 (max_steps spent), -2 (capacity), or NULL
+ * Returns MCL_EOVERFLOW when a cluster needs more than max_steps steps.
+ */
+int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
+                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                double* ratio, int32_t* status, mcl_stats* stats);
+
+/*
  * State of cluster (run_index, sim) as the kernel holds it after max_steps
  * steps (0 = the initial state): coordinates [nm] in float32, which electrons
  * and holes are left, the incrementally maintained nearest live hole per

@@ -136,6 +136,8 @@ _SIGNATURES = {
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
     "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
+    "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_int32, C.c_uint64,
+                              C.c_int64, _P, _P, C.POINTER(Stats)]),
     # surrogate network (include/mcl_surrogate.h)
     "mcl_mlp_create": (C.c_int, [_P, C.POINTER(MlpDesc), C.POINTER(_P)]),
     "mcl_mlp_destroy": (None, [_P]),
@@ -284,6 +286,26 @@ class Context:
                                     C.byref(stats))
         self.check(st, "mcl_run_event")
         return lum, t, er, steps, stats
+
+    def run_lab(self, runs: Sequence[RunParams], observations: Sequence[Sequence[float]],
+                exact_fill: bool = False, seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7):
+        """Lab experiments with dose filling (one run per experiment, its T_start,
+        T_rate, D and e_ratio_start).  Returns (ratio [clusters, max_obs] float64:
+        n_e / N_e at each observation time, status [clusters] int32, Stats)."""
+        arr = run_array(runs)
+        if len(observations) != len(runs):
+            raise ValueError("one observation list per run")
+        counts = np.array([len(o) for o in observations], dtype=np.int32)
+        obs = np.ascontiguousarray(np.concatenate([np.asarray(o, np.float64) for o in observations]))
+        total = sum(int(r.sims) for r in runs)
+        ratio = np.zeros((total, int(counts.max())), dtype=np.float64)
+        status = np.zeros(total, dtype=np.int32)
+        stats = Stats()
+        st = self.lib.mcl_run_lab(self.handle, arr, len(runs), _ptr(obs), _ptr(counts),
+                                  1 if exact_fill else 0, C.c_uint64(seed), int(max_steps),
+                                  _ptr(ratio), _ptr(status), C.byref(stats))
+        self.check(st, "mcl_run_lab")
+        return ratio, status, stats
 
     def cluster_state(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
                       max_steps: int = 0) -> dict:

@@ -202,12 +202,18 @@ __device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x,
     return key;
 }
 
+// w = exp(-alpha r) of a squared distance (every path computes it this one way).
+__device__ __forceinline__ float w_of(uint32_t d2bits, float alpha_f)
+{
+    return exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
+}
+
 __device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_t& slot, float& w)
 {
     const uint32_t d2bits = (uint32_t)(key >> 32);
     const bool none = d2bits >= 0x7f800000u;        // no live hole left
     slot = none ? kNoHole : (uint32_t)(key & 0xFFFFu);
-    w = none ? 0.0f : exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
+    w = none ? 0.0f : w_of(d2bits, alpha_f);
 }
 
 // Bit r set iff any lane has bit r of its row mask set (rows <= 64), by ballots.

@@ -637,6 +637,177 @@ int mcl_sync(void* ctx)
     return MCL_OK;
 }
 
+int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
+                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                double* ratio, int32_t* status, mcl_stats* stats)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!runs || !obs_times || !obs_count || !ratio)
+        return fail(c, MCL_EINVAL, "runs, obs_times, obs_count and ratio must not be null");
+    if (n_runs <= 0) return fail(c, MCL_EINVAL, "n_runs must be > 0");
+    if (max_steps <= 0) return fail(c, MCL_EINVAL, "max_steps must be > 0");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    std::vector<mcl::RunDesc> desc((size_t)n_runs);
+    std::vector<int32_t> offs((size_t)n_runs);
+    int64_t total = 0, n_obs_all = 0;
+    int max_obs = 0, max_e = 0, max_h = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        const mcl_run_params& p = runs[r];
+        if (!finite_all(p)) return fail(c, MCL_EINVAL, "run %d: non-finite parameter", r);
+        if (p.sims < 0) return fail(c, MCL_EINVAL, "run %d: sims must be >= 0", r);
+        if (!(p.D >= 0.0)) return fail(c, MCL_EINVAL, "run %d: dose rate D must be >= 0", r);
+        if (p.D > 0.0 && !(p.D0 > 0.0)) return fail(c, MCL_EINVAL, "run %d: D0 must be > 0", r);
+        if (!(p.k_b > 0.0) || !(p.N_e >= 1.0)) return fail(c, MCL_EINVAL, "run %d: k_b > 0, N_e >= 1", r);
+        if (!(p.alpha > 0.0) || !(p.rho_prime > 0.0) || !(p.holes > 0.0) || !(p.boundary_factor > 0.0))
+            return fail(c, MCL_EINVAL, "run %d: alpha, rho_prime, holes, boundary_factor must be > 0", r);
+        if (p.s < 0.0 || p.b < 0.0) return fail(c, MCL_EINVAL, "run %d: s and b must be >= 0", r);
+        const int k = obs_count[r];
+        if (k <= 0) return fail(c, MCL_EINVAL, "run %d: needs at least one observation time", r);
+        offs[(size_t)r] = (int32_t)n_obs_all;
+        for (int j = 0; j < k; ++j) {
+            const double o = obs_times[n_obs_all + j];
+            if (!std::isfinite(o) || o < 0.0 || (j > 0 && o < obs_times[n_obs_all + j - 1]))
+                return fail(c, MCL_EINVAL, "run %d: observation times must be finite, >= 0 and "
+                            "ascending", r);
+        }
+        const double t_end = obs_times[n_obs_all + k - 1];
+        if (!((p.T_start + 273.15) > 0.0) || !((p.T_start + 273.15) + p.T_rate * t_end > 0.0))
+            return fail(c, MCL_EINVAL, "run %d: temperature must stay > 0 K", r);
+        n_obs_all += k;
+        max_obs = std::max(max_obs, k);
+        mcl_geometry g;
+        geometry(p, g);
+        if (g.n_e0 < 0 || g.n_h < 0 || g.n_e0 > 8192 || g.n_h > 16384 || p.N_e > 8192)
+            return fail(c, MCL_ERANGE, "run %d: lab clusters hold at most 8192 electrons and "
+                        "16384 holes", r);
+        desc[(size_t)r] = make_desc(p, g, r, c->holes_per_cell);
+        total += p.sims;
+        if (p.sims > 0) {
+            max_e = std::max(max_e, std::max(g.n_e0, (int)p.N_e));
+            max_h = std::max(max_h, g.n_h);
+        }
+    }
+    if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
+    const int e_cap = ((max_e + 64 + 63) / 64) * 64;
+    const int h_cap = ((max_h + 2 * max_e + 64 + 3) / 4) * 4;
+    const int lds = mcl::lab_lds_bytes(h_cap, e_cap);
+    if (lds > c->max_lds)
+        return fail(c, MCL_ERANGE, "lab cluster needs %d B of LDS; a workgroup has %d B", lds,
+                    c->max_lds);
+    std::vector<int32_t> crun((size_t)total), csim((size_t)total);
+    {
+        int64_t k = 0;
+        for (int r = 0; r < n_runs; ++r)
+            for (int s2 = 0; s2 < runs[r].sims; ++s2, ++k) {
+                crun[(size_t)k] = r;
+                csim[(size_t)k] = s2;
+            }
+    }
+    std::vector<void*> tmp;
+    auto dmalloc = [&](void** ptr, size_t bytes) {
+        const hipError_t e = hipMalloc(ptr, bytes);
+        if (e == hipSuccess) tmp.push_back(*ptr);
+        return e;
+    };
+    auto release = [&]() {
+        for (void* q : tmp) (void)hipFree(q);
+    };
+    mcl::RunDesc* d_runs = nullptr;
+    int32_t *d_crun = nullptr, *d_csim = nullptr, *d_off = nullptr, *d_cnt = nullptr,
+            *d_status = nullptr;
+    double *d_obs = nullptr, *d_ratio = nullptr;
+    uint64_t* d_work = nullptr;
+    const size_t n_ratio = (size_t)total * (size_t)max_obs;
+    hipError_t e = dmalloc((void**)&d_runs, sizeof(mcl::RunDesc) * desc.size());
+    if (e == hipSuccess) e = dmalloc((void**)&d_crun, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_csim, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_off, sizeof(int32_t) * (size_t)n_runs);
+    if (e == hipSuccess) e = dmalloc((void**)&d_cnt, sizeof(int32_t) * (size_t)n_runs);
+    if (e == hipSuccess) e = dmalloc((void**)&d_status, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = dmalloc((void**)&d_obs, sizeof(double) * (size_t)n_obs_all);
+    if (e == hipSuccess) e = dmalloc((void**)&d_ratio, sizeof(double) * n_ratio);
+    if (e == hipSuccess) e = dmalloc((void**)&d_work, sizeof(uint64_t) * 4 * (size_t)total);
+    if (e != hipSuccess) {
+        release();
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "allocation: %s",
+                    hipGetErrorString(e));
+    }
+    auto h2d = [&](void* d, const void* h, size_t bytes) {
+        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream);
+    };
+    e = h2d(d_runs, desc.data(), sizeof(mcl::RunDesc) * desc.size());
+    if (e == hipSuccess) e = h2d(d_crun, crun.data(), sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = h2d(d_csim, csim.data(), sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = h2d(d_off, offs.data(), sizeof(int32_t) * (size_t)n_runs);
+    if (e == hipSuccess) e = h2d(d_cnt, obs_count, sizeof(int32_t) * (size_t)n_runs);
+    if (e == hipSuccess) e = h2d(d_obs, obs_times, sizeof(double) * (size_t)n_obs_all);
+    if (e == hipSuccess) e = mcl::prepare_lab(lds);
+    mcl::LabArgs a{};
+    a.runs = d_runs;
+    a.c_run = d_crun;
+    a.c_sim = d_csim;
+    a.n_clusters = (int32_t)total;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.obs = d_obs;
+    a.obs_offset = d_off;
+    a.obs_count = d_cnt;
+    a.max_obs = max_obs;
+    a.ratio = d_ratio;
+    a.status = d_status;
+    a.work = d_work;
+    a.max_steps = max_steps;
+    a.h_cap = h_cap;
+    a.e_cap = e_cap;
+    a.exact_fill = exact_fill ? 1 : 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    if (e == hipSuccess) e = hipEventRecord(ev0, c->stream);
+    if (e == hipSuccess) e = mcl::launch_lab(a, lds, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev1, c->stream);
+    std::vector<int32_t> hs((size_t)total);
+    std::vector<uint64_t> hw(4 * (size_t)total);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ratio, d_ratio, sizeof(double) * n_ratio, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hs.data(), d_status, sizeof(int32_t) * (size_t)total,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hw.data(), d_work, sizeof(uint64_t) * 4 * (size_t)total,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ev0, ev1);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    release();
+    if (e != hipSuccess)
+        return fail(c, MCL_EDEVICE, "lab kernel or copy failed: %s", hipGetErrorString(e));
+    if (status) std::memcpy(status, hs.data(), sizeof(int32_t) * (size_t)total);
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->kernel_ms = ms;
+        stats->clusters = (uint64_t)total;
+        for (int64_t k = 0; k < total; ++k) {
+            stats->live_electron_steps += hw[4 * (size_t)k];
+            stats->recombinations += hw[4 * (size_t)k + 2];
+            stats->cluster_steps += hw[4 * (size_t)k + 3];
+        }
+    }
+    for (int64_t k = 0; k < total; ++k) {
+        if (hs[(size_t)k] == -1)
+            return fail(c, MCL_EOVERFLOW, "cluster %lld (run %d): more than max_steps=%lld steps "
+                        "before the last observation", (long long)k, crun[(size_t)k],
+                        (long long)max_steps);
+        if (hs[(size_t)k] == -2)
+            return fail(c, MCL_ERANGE, "cluster %lld (run %d): electrons or holes beyond the lab "
+                        "cluster capacity", (long long)k, crun[(size_t)k]);
+    }
+    return MCL_OK;
+}
+
 int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* duration,
                   uint64_t seed, int32_t max_steps, uint32_t flags, uint8_t* lum, double* t_axis,
                   double* e_ratio, int32_t* steps, mcl_stats* stats)

@@ -86,6 +86,27 @@ struct EventArgs {
 };
 constexpr int32_t kEventFillUnsupported = -0x7fffffff;
 
+// Lab experiments (TL_lab / ISO_lab): event mode with dose filling, recording
+// n_e / N_e at observation times.  One cluster per workgroup.
+struct LabArgs {
+    const RunDesc* runs;
+    const int32_t* c_run;
+    const int32_t* c_sim;
+    int32_t n_clusters;
+    uint32_t key0, key1;
+    const double* obs;          // observation times of every run, concatenated (ascending per run)
+    const int32_t* obs_offset;  // [n_runs] first observation of each run
+    const int32_t* obs_count;   // [n_runs]
+    int32_t max_obs;            // ratio row stride
+    double* ratio;              // [n_clusters][max_obs]
+    int32_t* status;            // [n_clusters]: 0; -1 step budget spent; -2 capacity exceeded
+    uint64_t* work;             // [n_clusters][4] live electron-steps, fills, recombinations, steps
+    int64_t max_steps;
+    int32_t h_cap, e_cap;
+    int32_t exact_fill;         // 1: fills update every nearest hole exactly
+};
+int lab_lds_bytes(int h_cap, int e_cap);
+
 constexpr int kWaveHost = 64;   // wavefront width of gfx950
 
 // LDS bytes the fixed-dt kernel needs for the given capacities.
@@ -108,6 +129,8 @@ hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream);
 hipError_t prepare_fixed_dt(int lds_bytes);
 hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu);
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
+hipError_t launch_lab(const LabArgs& a, int lds_bytes, hipStream_t stream);
+hipError_t prepare_lab(int lds_bytes);
 hipError_t prepare_event(int lds_bytes);
 
 }  // namespace mcl

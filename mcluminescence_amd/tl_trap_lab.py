@@ -116,10 +116,12 @@ class TLTrapSim:
         counts, _, _ = self._context().run_fixed_dt(runs, seed=self.seed, n_bins=0)
         return counts[self.run_index * (self.sim + 1) + self.sim]
 
-    def TL_lab(self, csv_name: str, *, plot: bool = False) -> float:
+    def TL_lab(self, csv_name: str, *, plot: bool = False, **kw) -> float:
+        """MSE against a TL lab table (tl_trap_lab.py:65-118); keywords of lab.tl_lab."""
         from .lab import tl_lab
-        return tl_lab(self, csv_name, plot=plot)
+        return tl_lab(self, csv_name, plot=plot, **kw)
 
-    def ISO_lab(self, csv_name: str, *, plot: bool = False) -> float:
+    def ISO_lab(self, csv_name: str, *, plot: bool = False, **kw) -> float:
+        """MSE against an isothermal lab table (tl_trap_lab.py:120-179)."""
         from .lab import iso_lab
-        return iso_lab(self, csv_name, plot=plot)
+        return iso_lab(self, csv_name, plot=plot, **kw)

@@ -798,3 +798,241 @@ int orc_run_event(const mcl_run_params* runs, int32_t n_runs, const double* dura
     free(c_run); free(c_sim);
     return 0;
 }
+
+/* --------------------------------------------------------- lab experiments -- */
+/* Replay of the lab kernel (mcluminescence_amd/csrc/mc_lab.hip): TL_lab / ISO_lab
+ * loops (tl_trap_lab.py:65-179) with dose filling, electrons and holes in creation
+ * order with explicit float32 coordinates, nearest holes by full scan. */
+typedef struct {
+    float *hx, *hy, *hz;      /* hole slots; x = 3e38 marks a removed hole */
+    float *ex, *ey, *ez, *ew;
+    uint32_t* enn;
+    int n_e, n_hs, h_cap, e_cap;
+} lab_t;
+
+#define LAB_DEAD 3e38f
+
+static uint64_t lab_nearest(const lab_t* L, int n, float x, float y, float z)
+{
+    uint64_t key = ~0ull;
+    for (int s = 0; s < n; ++s) {
+        float d2;
+        if (L->hx[s] == LAB_DEAD) d2 = INFINITY;
+        else {
+            float dx = L->hx[s] - x, dy = L->hy[s] - y, dz = L->hz[s] - z;
+            d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        }
+        uint32_t b;
+        memcpy(&b, &d2, 4);
+        const uint64_t k = ((uint64_t)b << 32) | (uint32_t)s;
+        if (k < key) key = k;
+    }
+    return key;
+}
+
+static void lab_from_key(uint64_t key, float alpha_f, uint32_t* slot, float* w)
+{
+    const uint32_t b = (uint32_t)(key >> 32);
+    if (b >= 0x7f800000u) { *slot = MCL_NO_HOLE; *w = 0.0f; return; }
+    float d2;
+    memcpy(&d2, &b, 4);
+    *slot = (uint32_t)(key & 0xFFFFu);
+    *w = orc_expf(-(alpha_f * sqrtf(d2)));
+}
+
+static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const uint32_t key[2],
+                               const double* obs, int n_obs, int exact_fill, int64_t max_steps,
+                               int h_cap, int e_cap, double* out)
+{
+    mcl_geometry g;
+    orc_run_geometry(p, &g);
+    const double side_nm = g.side_m * 1e9, side_b_nm = (g.side_m * p->boundary_factor) * 1e9;
+    lab_t L = {0};
+    L.h_cap = h_cap; L.e_cap = e_cap;
+    L.hx = xcalloc(h_cap, 4); L.hy = xcalloc(h_cap, 4); L.hz = xcalloc(h_cap, 4);
+    L.ex = xcalloc(e_cap, 4); L.ey = xcalloc(e_cap, 4); L.ez = xcalloc(e_cap, 4);
+    L.ew = xcalloc(e_cap, 4); L.enn = xcalloc(e_cap, 4);
+    uint32_t* remap = xcalloc(h_cap, 4);
+    const uint32_t c3e = ((uint32_t)sim << 4) | MCL_TAG_GEOM_E, c3h = ((uint32_t)sim << 4) | MCL_TAG_GEOM_H;
+    const float alpha_f = (float)(p->alpha * 1e-9);
+    L.n_e = g.n_e0;
+    L.n_hs = g.n_h;
+    for (int h = 0; h < L.n_hs; ++h) {
+        uint32_t ctr[4] = {0u, (uint32_t)h, (uint32_t)run, c3h}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        L.hx[h] = (float)(((double)o[0] * 0x1p-32) * side_b_nm);
+        L.hy[h] = (float)(((double)o[1] * 0x1p-32) * side_b_nm);
+        L.hz[h] = (float)(((double)o[2] * 0x1p-32) * side_b_nm);
+    }
+    for (int e = 0; e < L.n_e; ++e) {
+        uint32_t ctr[4] = {0u, (uint32_t)e, (uint32_t)run, c3e}, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        L.ex[e] = (float)(((double)o[0] * 0x1p-32) * side_nm);
+        L.ey[e] = (float)(((double)o[1] * 0x1p-32) * side_nm);
+        L.ez[e] = (float)(((double)o[2] * 0x1p-32) * side_nm);
+    }
+    for (int e = 0; e < L.n_e; ++e)
+        lab_from_key(lab_nearest(&L, L.n_hs, L.ex[e], L.ey[e], L.ez[e]), alpha_f, &L.enn[e], &L.ew[e]);
+
+    const int two_ch = (p->E_loc_1 != p->E_loc_2) && (p->Retrap > 0.0) && (p->Retrap < 1.0);
+    const int always2 = p->Retrap >= 1.0;
+    const uint64_t ret_thr = two_ch ? (uint64_t)(p->Retrap * 0x1p32) : 0u;
+    const int b1_is_a = p->b == p->s && p->E_loc_1 == p->E_cb;
+    const int b2_is_b1 = p->E_loc_2 == p->E_loc_1;
+    const uint32_t c3f = ((uint32_t)sim << 4) | MCL_TAG_FIRE, c3c = ((uint32_t)sim << 4) | MCL_TAG_CHAN;
+    const int N_int = (int)p->N_e;
+    const double T0_K = p->T_start + 273.15;
+    double* rtot = xcalloc((size_t)(e_cap / 64 + 1), sizeof(double));
+    double v[64];
+    double t = 0.0;
+    int64_t i = 0;
+    int k_obs = 0, fills = 0;
+    int32_t status = 0;
+    while (k_obs < n_obs) {
+        if (i >= max_steps) { status = -1; break; }
+        const double T = T0_K + p->T_rate * t;
+        const double kT = p->k_b * T;
+        const double A = p->s * orc_exp(-(p->E_cb / kT));
+        const double B1 = b1_is_a ? A : p->b * orc_exp(-(p->E_loc_1 / kT));
+        const double B2 = b2_is_b1 ? B1 : p->b * orc_exp(-(p->E_loc_2 / kT));
+        const int rows = (L.n_e + 63) / 64;
+        double K = 0.0;
+#define LAB_RATE(pp) ((pp) >= L.n_e || L.enn[(pp)] == MCL_NO_HOLE ? 0.0 : \
+        fma(((two_ch ? (uint64_t)philox_word((uint32_t)(i >> 2), (uint32_t)(pp), (uint32_t)run, c3c, key, (int)(i & 3)) < ret_thr : always2) ? B2 : B1), \
+            (double)L.ew[(pp)], A))
+        for (int r = 0; r < rows; ++r) {
+            for (int l = 0; l < 64; ++l) v[l] = LAB_RATE(r * 64 + l);
+            scan64(v);
+            rtot[r] = v[63];
+            K = K + v[63];
+        }
+        uint32_t sb[4];
+        { uint32_t ctr[4] = {(uint32_t)i, 0xFFFFFFFFu, (uint32_t)run, c3f}; orc_philox4x32_10(ctr, key, sb); }
+        const double lam = (p->D == 0.0 || L.n_e == N_int) ? 1e-20 : (p->D / p->D0) * (double)(N_int - L.n_e);
+        const double dt_fill = (1.0 / lam) * -orc_log(((double)sb[2] + 0.5) * 0x1p-32);
+        double dt_rec = dt_fill;
+        if (L.n_e > 0) dt_rec = K > 0.0 ? -orc_log(((double)sb[0] + 0.5) * 0x1p-32) / K : INFINITY;
+        const double dt = fmin(dt_rec, dt_fill);
+        t += dt;
+        if (dt == dt_fill) {
+            if (L.n_e >= e_cap) { status = -2; break; }
+            if (L.n_hs >= h_cap) {
+                int base = 0;
+                for (int s = 0; s < L.n_hs; ++s) {
+                    if (L.hx[s] == LAB_DEAD) { remap[s] = MCL_NO_HOLE; continue; }
+                    remap[s] = (uint32_t)base;
+                    L.hx[base] = L.hx[s]; L.hy[base] = L.hy[s]; L.hz[base] = L.hz[s];
+                    ++base;
+                }
+                for (int e = 0; e < L.n_e; ++e) if (L.enn[e] != MCL_NO_HOLE) L.enn[e] = remap[L.enn[e]];
+                L.n_hs = base;
+                if (L.n_hs >= h_cap) { status = -2; break; }
+            }
+            uint32_t oe[4], oh[4];
+            { uint32_t ctr[4] = {(uint32_t)fills, 0xFFFFFFFEu, (uint32_t)run, c3e}; orc_philox4x32_10(ctr, key, oe); }
+            { uint32_t ctr[4] = {(uint32_t)fills, 0xFFFFFFFEu, (uint32_t)run, c3h}; orc_philox4x32_10(ctr, key, oh); }
+            const float ex = (float)(((double)oe[0] * 0x1p-32) * side_nm), ey = (float)(((double)oe[1] * 0x1p-32) * side_nm),
+                        ez = (float)(((double)oe[2] * 0x1p-32) * side_nm);
+            const int hs = L.n_hs;
+            L.hx[hs] = (float)(((double)oh[0] * 0x1p-32) * side_b_nm);
+            L.hy[hs] = (float)(((double)oh[1] * 0x1p-32) * side_b_nm);
+            L.hz[hs] = (float)(((double)oh[2] * 0x1p-32) * side_b_nm);
+            uint32_t slot;
+            float w;
+            lab_from_key(lab_nearest(&L, exact_fill ? hs + 1 : hs, ex, ey, ez), alpha_f, &slot, &w);
+            if (exact_fill) {
+                for (int e = 0; e < L.n_e; ++e) {
+                    float dx = L.hx[hs] - L.ex[e], dy = L.hy[hs] - L.ey[e], dz = L.hz[hs] - L.ez[e];
+                    float d2n = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    float d2o = INFINITY;
+                    const uint32_t s0 = L.enn[e];
+                    if (s0 != MCL_NO_HOLE) {
+                        float ox = L.hx[s0] - L.ex[e], oy = L.hy[s0] - L.ey[e], oz = L.hz[s0] - L.ez[e];
+                        d2o = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+                    }
+                    if (d2n < d2o) { L.enn[e] = (uint32_t)hs; L.ew[e] = orc_expf(-(alpha_f * sqrtf(d2n))); }
+                }
+            }
+            L.ex[L.n_e] = ex; L.ey[L.n_e] = ey; L.ez[L.n_e] = ez; L.ew[L.n_e] = w; L.enn[L.n_e] = slot;
+            L.n_e += 1; L.n_hs += 1; fills += 1;
+        } else if (dt == dt_rec) {
+            const double target = (double)sb[1] * 0x1p-32 * K;
+            int pw = -1;
+            double carry = 0.0;
+            for (int r = 0; r < rows && pw < 0; ++r) {
+                const double c2 = carry + rtot[r];
+                if (c2 > target) {
+                    for (int l = 0; l < 64; ++l) v[l] = LAB_RATE(r * 64 + l);
+                    scan64(v);
+                    for (int l = 0; l < 64; ++l) if (carry + v[l] > target) { pw = r * 64 + l; break; }
+                    break;
+                }
+                carry = c2;
+            }
+            if (pw < 0)
+                for (int s2 = L.n_e - 1; s2 >= 0 && pw < 0; --s2) if (LAB_RATE(s2) > 0.0) pw = s2;
+#undef LAB_RATE
+            const uint32_t hw = L.enn[pw];
+            L.hx[hw] = LAB_DEAD;
+            for (int q = pw + 1; q < L.n_e; ++q) {
+                L.ex[q - 1] = L.ex[q]; L.ey[q - 1] = L.ey[q]; L.ez[q - 1] = L.ez[q];
+                L.ew[q - 1] = L.ew[q]; L.enn[q - 1] = L.enn[q];
+            }
+            L.n_e -= 1;
+            for (int e = 0; e < L.n_e; ++e)
+                if (L.enn[e] == hw)
+                    lab_from_key(lab_nearest(&L, L.n_hs, L.ex[e], L.ey[e], L.ez[e]), alpha_f, &L.enn[e], &L.ew[e]);
+        }
+        ++i;
+        while (k_obs < n_obs && t >= obs[k_obs]) { out[k_obs] = (double)L.n_e / p->N_e; ++k_obs; }
+    }
+    free(rtot); free(remap);
+    free(L.hx); free(L.hy); free(L.hz); free(L.ex); free(L.ey); free(L.ez); free(L.ew); free(L.enn);
+    return status;
+}
+
+int orc_run_lab(const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
+                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                int32_t n_threads, double* ratio, int32_t* status)
+{
+    if (!runs || n_runs <= 0 || !obs_times || !obs_count || !ratio) return MCL_EINVAL;
+    int64_t total = 0;
+    int max_obs = 0, max_e = 0, max_h = 0;
+    int32_t* offs = xcalloc((size_t)n_runs, sizeof(int32_t));
+    int64_t o = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        offs[r] = (int32_t)o;
+        o += obs_count[r];
+        if (obs_count[r] > max_obs) max_obs = obs_count[r];
+        total += runs[r].sims;
+        if (runs[r].sims > 0) {
+            mcl_geometry g;
+            orc_run_geometry(&runs[r], &g);
+            int ne = g.n_e0 > (int)runs[r].N_e ? g.n_e0 : (int)runs[r].N_e;
+            if (ne > max_e) max_e = ne;
+            if (g.n_h > max_h) max_h = g.n_h;
+        }
+    }
+    const int e_cap = ((max_e + 64 + 63) / 64) * 64;
+    const int h_cap = ((max_h + 2 * max_e + 64 + 3) / 4) * 4;
+    int32_t* c_run = xcalloc((size_t)total, sizeof(int32_t));
+    int32_t* c_sim = xcalloc((size_t)total, sizeof(int32_t));
+    int64_t k = 0;
+    for (int r = 0; r < n_runs; ++r)
+        for (int s = 0; s < runs[r].sims; ++s) { c_run[k] = r; c_sim[k] = s; ++k; }
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t c = 0; c < total; ++c) {
+        double* out = ratio + (size_t)c * (size_t)max_obs;
+        memset(out, 0, sizeof(double) * (size_t)max_obs);
+        const int r = c_run[c];
+        const int32_t st = run_cluster_lab(&runs[r], r, c_sim[c], key, obs_times + offs[r],
+                                           obs_count[r], exact_fill, max_steps, h_cap, e_cap, out);
+        if (status) status[c] = st;
+    }
+    free(c_run); free(c_sim); free(offs);
+    return 0;
+}

@@ -92,6 +92,27 @@ def run_event(runs: Sequence[RunParams], duration, max_steps: int, seed: int = D
     return lum, t, er, steps
 
 
+def run_lab(runs: Sequence[RunParams], observations, exact_fill: bool = False,
+            seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7, threads: int = 0):
+    """Replay of the lab kernel: (ratio [clusters, max_obs], status)."""
+    L = lib()
+    L.orc_run_lab.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                              C.c_uint64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
+    L.orc_run_lab.restype = C.c_int
+    arr = run_array(runs)
+    counts = np.array([len(o) for o in observations], dtype=np.int32)
+    obs = np.ascontiguousarray(np.concatenate([np.asarray(o, np.float64) for o in observations]))
+    total = sum(int(r.sims) for r in runs)
+    ratio = np.zeros((total, int(counts.max())), dtype=np.float64)
+    status = np.zeros(total, dtype=np.int32)
+    st = L.orc_run_lab(arr, len(runs), obs.ctypes.data, counts.ctypes.data, 1 if exact_fill else 0,
+                       C.c_uint64(seed), int(max_steps), int(threads), ratio.ctypes.data,
+                       status.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_run_lab failed: {st}")
+    return ratio, status
+
+
 def log64(x: float) -> float:
     L = lib()
     L.orc_log.argtypes = [C.c_double]
