@@ -118,13 +118,15 @@ __device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, fl
     return key;
 }
 
-// Wave-cooperative search for one electron (x, y, z wave-uniform): the lanes
-// spread over the concatenated slot ranges of the 3x3x3 block, one hole each,
-// then a wave min.  About 150 VALU instructions per electron against ~1000 for
-// a lane-private pass, so it wins while few electrons are queued.
+// Wave-cooperative search for one electron (x, y, z wave-uniform).  The 3x3x3
+// block is at most 9 contiguous slot ranges (one per (z, y) row of cells); lane
+// l takes range l / 7 at offsets l % 7, l % 7 + 7, ... so every lane finds its
+// slots from two cell starts, with no cross-lane bookkeeping, then a wave min.
+// Ranges longer than 7 holes cost one more pass each 7.
 __device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, float y, float z,
                                                       int lane, bool& conclusive, uint64_t& evals)
 {
+    constexpr int kPer = 7;                          // lanes per range (9 x 7 = 63 lanes)
     const int G = g.G;
     const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
               cz = cell_of(z, g.inv_cell, G);
@@ -133,29 +135,20 @@ __device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, fl
     const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
     const int ny = y1 - y0 + 1;
     const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
-    int s = 0, len = 0;
-    if (lane < nr) {
-        const int zz = z0 + lane / ny, yy = y0 + lane % ny;
+    const int j = lane / kPer, o = lane - j * kPer;
+    int s = 0, e = 0;
+    if (j < nr) {
+        const int zz = z0 + j / ny, yy = y0 + j % ny;
         const int base = (zz * G + yy) * G;
         s = (int)g.cstart[base + x0];
-        len = (int)g.cstart[base + x1 + 1] - s;
+        e = (int)g.cstart[base + x1 + 1];
     }
-    int incl = len;                                  // inclusive prefix over lanes 0..15
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-        const int v = __shfl_up(incl, off);
-        if (lane >= off) incl += v;
-    }
-    const int total = __shfl(incl, 15);
-    const int excl = incl - len;
-    if (lane == 0) evals += (uint64_t)total;         // summed over lanes at the end
+    if (o == 0) evals += (uint64_t)(e - s);          // summed over lanes at the end
     uint64_t key = ~0ull;
-    for (int l0 = 0; l0 < total; l0 += kWave) {
-        const int l = l0 + lane;
-        int sel = 0;
-        for (int j = 1; j < nr; ++j) sel = (l >= __shfl(excl, j)) ? j : sel;
-        const int slot = __shfl(s, sel) + (l - __shfl(excl, sel));
-        if (l < total) {
+    for (int slot = s + o;; slot += kPer) {
+        const bool v = slot < e;
+        if (ballot(v) == 0) break;
+        if (v) {
             const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
             key = k < key ? k : key;
         }
