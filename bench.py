@@ -89,6 +89,7 @@ def tl_extras(ctx, stream) -> dict:
         plan.sync()
         ms.append(plan.last_ms())
     plan.close()
+    glow = glow_extras(ctx, W.tl_glow_16x500(), counts)
     theta = S.sample_params(1024, np.random.default_rng(3))
     S.mc_curves(ctx, theta[:64], n=400, sims=8)
     torch.cuda.synchronize()
@@ -97,9 +98,37 @@ def tl_extras(ctx, stream) -> dict:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"tl_glow_16x500_ms": float(np.median(ms)),
+            "glow_curve": glow,
             "tl_curves_per_s": 1024 / dt,
             "tl_curves_note": "256-bin glow curves of 8 clusters x 400 electrons each, 2048 steps, "
                               "1024 parameter sets per call, host round trip included"}
+
+
+def glow_extras(ctx, runs, counts, batch: int = 1024) -> dict:
+    """The glow-curve kernel (1-degree bins, 50-degree boxcar) on the 16 x 500 glow's
+    per-step counts, alone and as a batch of `batch` copies of that run (HBM-bound:
+    algorithmic bytes = clusters x steps x 4 B of counts + the curves)."""
+    import numpy as np
+    import torch
+    out = {}
+    for label, reps in (("one_run", 1), (f"batch_{batch}", batch)):
+        c = counts.repeat(reps, 1).contiguous()
+        rr = runs * reps
+        curve = torch.zeros((reps, 512), dtype=torch.float64, device="cuda")
+        lens = torch.zeros(reps, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(6):
+            st = ctx.glow_curve_device(rr, c.data_ptr(), 4, c.shape[1], None, 1.0, 50.0,
+                                       curve.data_ptr(), 512, lens.data_ptr())
+            ms.append(st.kernel_ms)
+        m = float(np.median(ms[1:]))
+        nbytes = c.numel() * 4 + int(lens.sum().item()) * 8
+        out[label] = {"runs": reps, "clusters": int(c.shape[0]), "steps": int(c.shape[1]),
+                      "points": int(lens[0].item()), "kernel_ms": m,
+                      "hbm_gbps": nbytes / (m * 1e-3) / 1e9,
+                      "frac": nbytes / (m * 1e-3) / 1e9 / PEAK_HBM_GBPS}
+    return out
 
 
 def surrogate_extras(ctx, fits: bool = True) -> dict:

@@ -113,7 +113,7 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     what fits on the device at once (default 1).
  *   "wave_search_max" electrons needing a new nearest hole in one step are
  *                     searched one at a time by the whole wave up to this
- *                     many, lane-parallel beyond (default 2, at most 64).
+ *                     many, lane-parallel beyond (default 4, at most 64).
  *   "lookahead_max"   with at most this many live electrons in a cluster, the
  *                     wave tests the next 64 steps at once and skips those in
  *                     which no electron can fire (default 8; 0 turns it off).
@@ -232,6 +232,37 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
 int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
                 const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
                 double* ratio, int32_t* status, mcl_stats* stats);
+
+/*
+ * Glow curves, one per run: the binning and smoothing of plots.py:39-48
+ * (hist_and_smooth) on the data path of plot_TL (plots.py:50-66), computed from
+ * the per-step counts of a sweep.  For run r with S = sims and first cluster c0:
+ *   t_i    t_axis[c0][i] (the first sim's time axis, plots.py:59), or with t_axis
+ *          NULL the fixed-dt axis (i + 1) dt for i < n_steps and 0 beyond
+ *   w_i    (sum of count[c][i] over the run's clusters) / S   (plots.py:60)
+ *   T_i    T_start + T_rate t_i for every step with t_i > 0  (plots.py:61-63)
+ *   hist   weighted histogram of T_i over the edges j * bin_width, j = 0 ..
+ *          ceil((max T + bin_width) / bin_width) - 1 (np.arange), bins
+ *          [e_j, e_j+1) with the last one closed, divided by bin_width
+ *   curve  boxcar mean over k = max(1, int(win / bin_width)) bins, 'valid' mode
+ *          (np.convolve with ones(k) / k, plots.py:20-22; when k exceeds the bin
+ *          count every point is the total over k)
+ *   counts     [total_clusters][row_stride]: u32 (count_bytes 4, fixed-dt counts)
+ *              or u8 (count_bytes 1, event-mode lum)
+ *   t_axis     [total_clusters][row_stride] f64, or NULL
+ *   curve      [n_runs][curve_stride] f64; curve_len [n_runs] points written (0
+ *              where no step has t > 0 or every T is <= 0: the reference raises)
+ * With MCL_F_DEVICE_PTRS counts, t_axis, curve and curve_len are device
+ * pointers; the call waits for the kernel either way.  A histogram with more
+ * than MCL_GLOW_MAX_BINS bins fails with MCL_ERANGE, a curve longer than
+ * curve_stride with MCL_EOVERFLOW (curve_len then holds the lengths needed).
+ * Sums over sims are exact (64-bit integers); each point is one division.
+ */
+#define MCL_GLOW_MAX_BINS 8192
+int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_runs, const void* counts,
+                   int32_t count_bytes, int64_t row_stride, const double* t_axis,
+                   double bin_width, double win, uint32_t flags, double* curve,
+                   int64_t curve_stride, int32_t* curve_len, mcl_stats* stats);
 
 /*
  * State of cluster (run_index, sim) as the kernel holds it after max_steps

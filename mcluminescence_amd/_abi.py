@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from pathlib import Path
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -29,6 +29,8 @@ MCL_EOVERFLOW = -75
 F_DEVICE_PTRS = 0x1
 F_PERSISTENT = 0x2
 F_GRAPH = 0x4
+
+MAX_GLOW_BINS = 8192          # MCL_GLOW_MAX_BINS
 
 TAG_GEOM_E = 0
 TAG_GEOM_H = 1
@@ -138,6 +140,9 @@ _SIGNATURES = {
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
     "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_int32, C.c_uint64,
                               C.c_int64, _P, _P, C.POINTER(Stats)]),
+    "mcl_glow_curve": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_int32, C.c_int64,
+                                 _P, C.c_double, C.c_double, C.c_uint32, _P, C.c_int64, _P,
+                                 C.POINTER(Stats)]),
     # surrogate network (include/mcl_surrogate.h)
     "mcl_mlp_create": (C.c_int, [_P, C.POINTER(MlpDesc), C.POINTER(_P)]),
     "mcl_mlp_destroy": (None, [_P]),
@@ -306,6 +311,62 @@ class Context:
                                   _ptr(ratio), _ptr(status), C.byref(stats))
         self.check(st, "mcl_run_lab")
         return ratio, status, stats
+
+    def glow_curve(self, runs: Sequence[RunParams], counts: np.ndarray,
+                   t_axis: Optional[np.ndarray] = None, bin_width: float = 1.0,
+                   win: float = 50.0) -> Tuple[List[np.ndarray], Stats]:
+        """Glow curves (mcl_glow_curve) from host arrays: counts [clusters, steps]
+        uint32 (fixed-dt counts) or uint8 (event lum), t_axis [clusters, steps]
+        float64 or None (fixed-dt axis (i + 1) dt of each run).  Returns one float64
+        curve per run and the Stats of the last launch."""
+        arr = run_array(runs)
+        counts = np.ascontiguousarray(counts)
+        if counts.dtype not in (np.uint32, np.uint8) or counts.ndim != 2:
+            raise ValueError("counts must be a 2-D uint32 or uint8 array")
+        total = sum(int(r.sims) for r in runs)
+        if counts.shape[0] != total:
+            raise ValueError(f"counts has {counts.shape[0]} rows for {total} clusters")
+        stride = counts.shape[1]
+        if t_axis is not None:
+            t_axis = np.ascontiguousarray(t_axis, dtype=np.float64)
+            if t_axis.shape != counts.shape:
+                raise ValueError("t_axis must have the shape of counts")
+        n = len(runs)
+        # first guess from the temperature span; a longer curve is reported and retried
+        span = max(abs(float(r.T_start)) + abs(float(r.T_rate)) * float(r.n_steps) * float(r.dt)
+                   for r in runs)
+        k = max(1, int(win / bin_width)) if bin_width > 0 else 1
+        guess = int(min(span / bin_width, 4 * MAX_GLOW_BINS)) + k + 4 if bin_width > 0 else 16
+        for _ in range(2):
+            curve = np.zeros((n, max(guess, 1)), dtype=np.float64)
+            lens = np.zeros(n, dtype=np.int32)
+            stats = Stats()
+            st = self.lib.mcl_glow_curve(self.handle, arr, n, _ptr(counts), counts.itemsize, stride,
+                                         _ptr(t_axis) if t_axis is not None else None,
+                                         float(bin_width), float(win), 0, _ptr(curve),
+                                         curve.shape[1], _ptr(lens), C.byref(stats))
+            if st == MCL_EOVERFLOW:
+                guess = int(lens.max())
+                continue
+            self.check(st, "mcl_glow_curve")
+            return [curve[r, :lens[r]].copy() for r in range(n)], stats
+        self.check(st, "mcl_glow_curve")
+        raise AssertionError("unreachable")
+
+    def glow_curve_device(self, runs: Sequence[RunParams], counts_ptr: int, count_bytes: int,
+                          row_stride: int, t_axis_ptr: Optional[int], bin_width: float,
+                          win: float, curve_ptr: int, curve_stride: int, len_ptr: int) -> Stats:
+        """mcl_glow_curve on device buffers (MCL_F_DEVICE_PTRS); waits for the kernel."""
+        arr = run_array(runs)
+        stats = Stats()
+        st = self.lib.mcl_glow_curve(self.handle, arr, len(runs), C.c_void_p(counts_ptr),
+                                     int(count_bytes), int(row_stride),
+                                     C.c_void_p(t_axis_ptr) if t_axis_ptr else None,
+                                     float(bin_width), float(win), F_DEVICE_PTRS,
+                                     C.c_void_p(curve_ptr), int(curve_stride),
+                                     C.c_void_p(len_ptr), C.byref(stats))
+        self.check(st, "mcl_glow_curve")
+        return stats
 
     def cluster_state(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
                       max_steps: int = 0) -> dict:

@@ -29,10 +29,10 @@ from typing import Dict, Iterable, List, Optional, Sequence
 from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geometry
 
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
-DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 2.0,
+DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 4.0,
             "lookahead_max": 8.0}
 SPACE = {"holes_per_cell": [2.0, 3.0, 4.0], "persist_frac": [1.0, 2.0],
-         "wave_search_max": [0.0, 2.0, 4.0], "lookahead_max": [0.0, 8.0, 32.0]}
+         "wave_search_max": [2.0, 4.0, 8.0], "lookahead_max": [0.0, 8.0, 32.0]}
 
 
 def signature(runs: Sequence[RunParams]) -> str:

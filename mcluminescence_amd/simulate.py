@@ -10,6 +10,11 @@
   * it writes ``results/simulations/exp_{tag}.csv`` with columns
     ``run,sim,step,lum,electron_ratio``, run-major then sim then step.
 
+The reference's CSV has no time axis (x_ax is only returned in memory,
+simulate.py:112), so a side-car ``exp_{tag}_x_ax.csv`` with columns
+``run,sim,step,t`` in the same row order is written beside it;
+``read_results(tag)`` rebuilds (x_ax, Lum, electron_ratio) from the pair.
+
 What runs underneath is one GPU launch for every (run, sim) cluster, in one of
 two modes (config key ``mode``, default ``fixed_dt``):
 
@@ -128,6 +133,50 @@ def write_results_csv(path: Path, lum: np.ndarray, e_ratio: np.ndarray) -> Path:
     return path
 
 
+def x_ax_path(results_csv: Path) -> Path:
+    """The time-axis side-car of results/simulations/exp_{tag}.csv."""
+    results_csv = Path(results_csv)
+    return results_csv.with_name(results_csv.stem + "_x_ax.csv")
+
+
+def write_x_ax_csv(path: Path, x_ax: np.ndarray) -> Path:
+    """Columns run,sim,step,t, the row order of the results CSV."""
+    import pandas as pd
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    steps, sims, runs = x_ax.shape
+    pd.DataFrame({
+        "run": np.repeat(np.arange(runs), sims * steps),
+        "sim": np.tile(np.repeat(np.arange(sims), steps), runs),
+        "step": np.tile(np.arange(steps), runs * sims),
+        "t": x_ax.transpose(2, 1, 0).ravel(),
+    }).to_csv(path, index=False)
+    return path
+
+
+def read_results(tag: str, *, results_dir: Optional[Path] = None):
+    """(x_ax, Lum, electron_ratio) as [steps, sims, runs] float64 from
+    exp_{tag}.csv and its side-car (x_ax is zero without one)."""
+    import pandas as pd
+    base = Path(results_dir) if results_dir is not None else paths.simulations_dir()
+    main = base / f"exp_{tag}.csv"
+    df = pd.read_csv(main, float_precision="round_trip")
+    runs, sims, steps = int(df.run.max()) + 1, int(df.sim.max()) + 1, int(df.step.max()) + 1
+    if len(df) != runs * sims * steps:
+        raise ValueError(f"{main} does not hold a full run x sim x step grid")
+    shape = (runs, sims, steps)
+    lum = df.lum.to_numpy(np.float64).reshape(shape).transpose(2, 1, 0).copy()
+    e_ratio = df.electron_ratio.to_numpy(np.float64).reshape(shape).transpose(2, 1, 0).copy()
+    side = x_ax_path(main)
+    x_ax = np.zeros_like(lum)
+    if side.exists():
+        tf = pd.read_csv(side, float_precision="round_trip")
+        if len(tf) != len(df):
+            raise ValueError(f"{side} does not match {main}")
+        x_ax = tf.t.to_numpy(np.float64).reshape(shape).transpose(2, 1, 0).copy()
+    return x_ax, lum, e_ratio
+
+
 def simulate(cfg: Config, *, ctx: Optional[Context] = None, seed: int = DEFAULT_SEED,
              write_csv: bool = True, results_dir: Optional[Path] = None
              ) -> Tuple[np.ndarray, np.ndarray, np.ndarray, Dict[int, Config]]:
@@ -145,7 +194,8 @@ def simulate(cfg: Config, *, ctx: Optional[Context] = None, seed: int = DEFAULT_
     if write_csv:
         tag = cfg.get("tag", "")
         out_dir = Path(results_dir) if results_dir is not None else paths.simulations_dir()
-        write_results_csv(out_dir / f"exp_{tag}.csv", lum, e_ratio)
+        main = write_results_csv(out_dir / f"exp_{tag}.csv", lum, e_ratio)
+        write_x_ax_csv(x_ax_path(main), x_ax)
     return x_ax, lum, e_ratio, configs
 
 
