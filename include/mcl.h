@@ -218,19 +218,42 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
  * e_ratio_start the initial fill; dt and n_steps are not used.  Events as in
  * mcl_run_event without a step cap, plus filling at lambda = (D/D0)(N_e - n_e)
  * (tl_trap_lab.py:53-60): one electron-hole pair per fill (Box.add_electron,
- * engine.py:133-152).  With exact_fill = 0 the new electron's nearest hole is
- * searched among the older holes and no other electron looks at the new hole,
- * the reference's update; exact_fill = 1 updates every nearest hole exactly.
+ * engine.py:133-152).  Without MCL_LAB_EXACT_FILL the new electron's nearest hole
+ * is searched among the older holes and no other electron looks at the new hole,
+ * the reference's update; with it every nearest hole is exact after a fill.
  * The trapped fraction n_e / N_e after the first step that reaches each
  * observation time is written to ratio.
  *   obs_times  all runs' observation times concatenated, ascending within a run
  *   obs_count  [n_runs] observations per run (>= 1)
+ *   options    MCL_LAB_* bits below (0: the class-based loops of tl_trap_lab.py)
  *   ratio      [total_clusters][max over runs of obs_count]
  *   status     [total_clusters] 0, -1 (max_steps spent), -2 (capacity), or NULL
  * Returns MCL_EOVERFLOW when a cluster needs more than max_steps steps.
+ *
+ * The other bits reproduce the legacy lab loops sim_lab_TL_residuals /
+ * sim_lab_TL_residuals_iso (src/est_params/functions.py:270-434):
+ *   MCL_LAB_SHARED_CHANNEL   one localisation-channel draw per step shared by all
+ *                            electrons: E_loc_2 iff U <= Retrap (functions.py:125-128)
+ *   MCL_LAB_RETRAP           after each recombination, with probability Retrap, one
+ *                            new electron-hole pair (functions.py:328-331; the TL loop)
+ *   MCL_LAB_FIRST_ELECTRON   the recombining electron is the oldest one (creation
+ *                            order) instead of one drawn with probability k_e / K:
+ *                            recomber as those loops call it (functions.py:241-261,
+ *                            its np.where over a (1, n_e) mask yields index 0)
+ *   MCL_LAB_LEGACY_GEOMETRY  holes = int(holes) + int(int(holes)/d^3 * ((bf d)^3 - d^3))
+ *                            with d = (int(holes)/rho)^(1/3) (initialize_box_bg,
+ *                            functions.py:51-80)
+ *   MCL_LAB_LEGACY           all of these plus MCL_LAB_EXACT_FILL (the legacy fill
+ *                            recomputes every nearest hole, functions.py:150-161)
  */
+#define MCL_LAB_EXACT_FILL       0x1u
+#define MCL_LAB_SHARED_CHANNEL   0x2u
+#define MCL_LAB_RETRAP           0x4u
+#define MCL_LAB_FIRST_ELECTRON   0x8u
+#define MCL_LAB_LEGACY_GEOMETRY  0x10u
+#define MCL_LAB_LEGACY           0x1Fu
 int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
-                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                const int32_t* obs_count, uint32_t options, uint64_t seed, int64_t max_steps,
                 double* ratio, int32_t* status, mcl_stats* stats);
 
 /*

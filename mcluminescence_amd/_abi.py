@@ -32,6 +32,14 @@ F_GRAPH = 0x4
 
 MAX_GLOW_BINS = 8192          # MCL_GLOW_MAX_BINS
 
+# mcl_run_lab options (MCL_LAB_*)
+LAB_EXACT_FILL = 0x1
+LAB_SHARED_CHANNEL = 0x2
+LAB_RETRAP = 0x4
+LAB_FIRST_ELECTRON = 0x8
+LAB_LEGACY_GEOMETRY = 0x10
+LAB_LEGACY = 0x1F
+
 TAG_GEOM_E = 0
 TAG_GEOM_H = 1
 TAG_FIRE = 2
@@ -138,7 +146,7 @@ _SIGNATURES = {
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
     "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
-    "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_int32, C.c_uint64,
+    "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_uint32, C.c_uint64,
                               C.c_int64, _P, _P, C.POINTER(Stats)]),
     "mcl_glow_curve": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_int32, C.c_int64,
                                  _P, C.c_double, C.c_double, C.c_uint32, _P, C.c_int64, _P,
@@ -293,10 +301,12 @@ class Context:
         return lum, t, er, steps, stats
 
     def run_lab(self, runs: Sequence[RunParams], observations: Sequence[Sequence[float]],
-                exact_fill: bool = False, seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7):
+                exact_fill: bool = False, seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7,
+                options: int = 0):
         """Lab experiments with dose filling (one run per experiment, its T_start,
-        T_rate, D and e_ratio_start).  Returns (ratio [clusters, max_obs] float64:
-        n_e / N_e at each observation time, status [clusters] int32, Stats)."""
+        T_rate, D and e_ratio_start).  options: LAB_* bits (exact_fill sets
+        LAB_EXACT_FILL).  Returns (ratio [clusters, max_obs] float64: n_e / N_e at
+        each observation time, status [clusters] int32, Stats)."""
         arr = run_array(runs)
         if len(observations) != len(runs):
             raise ValueError("one observation list per run")
@@ -306,8 +316,9 @@ class Context:
         ratio = np.zeros((total, int(counts.max())), dtype=np.float64)
         status = np.zeros(total, dtype=np.int32)
         stats = Stats()
+        opts = int(options) | (LAB_EXACT_FILL if exact_fill else 0)
         st = self.lib.mcl_run_lab(self.handle, arr, len(runs), _ptr(obs), _ptr(counts),
-                                  1 if exact_fill else 0, C.c_uint64(seed), int(max_steps),
+                                  opts, C.c_uint64(seed), int(max_steps),
                                   _ptr(ratio), _ptr(status), C.byref(stats))
         self.check(st, "mcl_run_lab")
         return ratio, status, stats

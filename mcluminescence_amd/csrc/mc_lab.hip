@@ -26,6 +26,18 @@
 //   again (engine.py:154-175).
 //   After the step every observation time <= t not yet passed records n_e / N_e
 //   (tl_trap_lab.py:163-172); the run ends when all are recorded.
+//
+// Options of the legacy lab loops (src/est_params/functions.py:270-434):
+//   MCL_LAB_SHARED_CHANNEL  one channel draw per step for every electron
+//                           (lifetime_fading, functions.py:115-135), word i & 3 of
+//                           Philox (i >> 2, 0xFFFFFFFD, run, sim|CHAN);
+//   MCL_LAB_FIRST_ELECTRON  the recombining electron is the first one in creation
+//                           order that has a hole (recomber as the lab loops call it,
+//                           functions.py:241-261: its np.where over a (1, n_e) mask
+//                           returns row indices, all 0);
+//   MCL_LAB_RETRAP          after a recombination, with probability Retrap (word w
+//                           of the step block), one more pair with exact nearest
+//                           holes (functions.py:328-331).
 #include "mc_common.h"
 
 namespace mcl {
@@ -33,6 +45,7 @@ namespace mcl {
 namespace {
 
 constexpr uint32_t kFillCtr = 0xFFFFFFFEu;
+constexpr uint32_t kSharedChanCtr = 0xFFFFFFFDu;
 
 __device__ __forceinline__ double lab_scan(double v, int lane)
 {
@@ -118,6 +131,10 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
 
     const bool two_ch = R.two_channel != 0;
     const bool always2 = R.always_chan2 != 0;
+    const bool exact_fill = (a.options & MCL_LAB_EXACT_FILL) != 0;
+    const bool shared_ch = (a.options & MCL_LAB_SHARED_CHANNEL) != 0;
+    const bool first_e = (a.options & MCL_LAB_FIRST_ELECTRON) != 0;
+    const bool retrap = (a.options & MCL_LAB_RETRAP) != 0;
     const uint64_t ret_thr = R.ret_thr;
     const uint32_t c3f = psim | MCL_TAG_FIRE, c3c = psim | MCL_TAG_CHAN;
     const bool b1_is_a = R.b == R.s && R.E_loc_1 == R.E_cb;
@@ -143,37 +160,21 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
         const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
         const uint32_t blk = (uint32_t)i >> 2;
         const int word = i & 3;
+        bool ch_step = always2;
+        if (two_ch && shared_ch)
+            ch_step = (uint64_t)pick_word(philox10(blk, kSharedChanCtr, prun, c3c, k0, k1), word) <
+                      ret_thr;
         auto rate_of = [&](int p) -> double {
             if (p >= n_e || enn[p] == kNoHole) return 0.0;
-            bool ch = always2;
-            if (two_ch)
+            bool ch = ch_step;
+            if (two_ch && !shared_ch)
                 ch = (uint64_t)pick_word(philox10(blk, (uint32_t)p, prun, c3c, k0, k1), word) <
                      ret_thr;
             return __builtin_fma(ch ? B2 : B1, (double)elec[p].w, A);
         };
-        const int rows = (n_e + kWave - 1) / kWave;
-        double K = 0.0;
-        for (int r = 0; r < rows; ++r) {
-            const double v = lab_scan(rate_of(r * kWave + lane), lane);
-            const double rt = __shfl(v, kWave - 1, kWave);
-            if (lane == 0) rtot[r] = rt;
-            K = K + rt;
-        }
-        __syncthreads();
-        const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
-        const double lam = (R.D == 0.0 || n_e == N_int) ? 1e-20
-                                                         : (R.D / R.D0) * (double)(N_int - n_e);
-        const double dt_fill = (1.0 / lam) * lab_exp1(sb.z);
-        double dt_rec = dt_fill;
-        if (n_e > 0) dt_rec = K > 0.0 ? lab_exp1(sb.x) / K : __builtin_inf();
-        const double dt = fmin(dt_rec, dt_fill);
-        t += dt;
-        if (dt == dt_fill) {
-            // ---- filling: one electron-hole pair (Box.add_electron) ------------
-            if (n_e >= e_cap) {
-                status = -2;
-                break;
-            }
+        // one electron-hole pair (Box.add_electron); false when the cluster is full
+        auto add_pair = [&](bool exact) -> bool {
+            if (n_e >= e_cap) return false;
             if (n_hs >= h_cap) {
                 // stable compaction of the hole slots (creation order kept)
                 int base = 0;
@@ -193,10 +194,7 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                     if (enn[e] != kNoHole) enn[e] = remap[enn[e]];
                 n_hs = base;
                 __syncthreads();
-                if (n_hs >= h_cap) {
-                    status = -2;
-                    break;
-                }
+                if (n_hs >= h_cap) return false;
             }
             const uint32_t f = (uint32_t)fills;
             const u32x4 oe = philox10(f, kFillCtr, prun, psim | MCL_TAG_GEOM_E, k0, k1);
@@ -211,11 +209,11 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
             if (lane == 0) hole[hs] = hn;
             __syncthreads();
             // the new electron's nearest hole: before the new hole (reference) or all
-            const uint64_t key = lab_nearest(hole, a.exact_fill ? hs + 1 : hs, ex, ey, ez, lane);
+            const uint64_t key = lab_nearest(hole, exact ? hs + 1 : hs, ex, ey, ez, lane);
             uint32_t slot;
             float w;
             nn_from_key(key, alpha_f, slot, w);
-            if (a.exact_fill) {
+            if (exact) {
                 // existing electrons: the new hole wins only when strictly closer
                 for (int e = lane; e < n_e; e += kWave) {
                     const float4 p = elec[e];
@@ -243,20 +241,53 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
             n_e += 1;
             n_hs += 1;
             fills += 1;
+            return true;
+        };
+        const int rows = (n_e + kWave - 1) / kWave;
+        double K = 0.0;
+        for (int r = 0; r < rows; ++r) {
+            const double v = lab_scan(rate_of(r * kWave + lane), lane);
+            const double rt = __shfl(v, kWave - 1, kWave);
+            if (lane == 0) rtot[r] = rt;
+            K = K + rt;
+        }
+        __syncthreads();
+        const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
+        const double lam = (R.D == 0.0 || n_e == N_int) ? 1e-20
+                                                         : (R.D / R.D0) * (double)(N_int - n_e);
+        const double dt_fill = (1.0 / lam) * lab_exp1(sb.z);
+        double dt_rec = dt_fill;
+        if (n_e > 0) dt_rec = K > 0.0 ? lab_exp1(sb.x) / K : __builtin_inf();
+        const double dt = fmin(dt_rec, dt_fill);
+        t += dt;
+        if (dt == dt_fill) {
+            // ---- filling: one electron-hole pair (Box.add_electron) ------------
+            if (!add_pair(exact_fill)) {
+                status = -2;
+                break;
+            }
         } else if (dt == dt_rec) {
             // ---- recombination (Box.remove_pair) --------------------------------
-            const double target = (double)sb.y * 0x1p-32 * K;
             int pw = -1;
-            double carry = 0.0;
-            for (int r = 0; r < rows; ++r) {
-                const double c2 = carry + rtot[r];
-                if (c2 > target) {
-                    const double v = lab_scan(rate_of(r * kWave + lane), lane);
-                    const uint64_t m = ballot(carry + v > target);
-                    pw = r * kWave + (int)__builtin_ctzll(m);
-                    break;
+            if (first_e) {
+                for (int r = 0; r < rows && pw < 0; ++r) {
+                    const int p = r * kWave + lane;
+                    const uint64_t m = ballot(p < n_e && enn[p] != kNoHole);
+                    if (m != 0) pw = r * kWave + (int)__builtin_ctzll(m);
                 }
-                carry = c2;
+            } else {
+                const double target = (double)sb.y * 0x1p-32 * K;
+                double carry = 0.0;
+                for (int r = 0; r < rows; ++r) {
+                    const double c2 = carry + rtot[r];
+                    if (c2 > target) {
+                        const double v = lab_scan(rate_of(r * kWave + lane), lane);
+                        const uint64_t m = ballot(carry + v > target);
+                        pw = r * kWave + (int)__builtin_ctzll(m);
+                        break;
+                    }
+                    carry = c2;
+                }
             }
             if (pw < 0) {
                 for (int r = rows - 1; r >= 0 && pw < 0; --r) {
@@ -296,6 +327,11 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                     elec[e].w = w;
                 }
                 __syncthreads();
+            }
+            // legacy retrapping: a new pair after the recombination
+            if (retrap && (uint64_t)sb.w < R.retrap_thr && !add_pair(true)) {
+                status = -2;
+                break;
             }
         }
         ++i;

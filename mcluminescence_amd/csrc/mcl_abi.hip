@@ -70,6 +70,21 @@ void geometry(const mcl_run_params& p, mcl_geometry& g)
     g.n_h = (int32_t)(p.holes * std::pow(p.boundary_factor, 3.0));
 }
 
+// The legacy stack's box (initialize_box_bg, functions.py:51-80): int(holes) core
+// holes set the side, and the boundary shell adds int(density * shell volume).
+void legacy_geometry(const mcl_run_params& p, mcl_geometry& g)
+{
+    g.rho = p.rho_prime * (3.0 / (4.0 * kPi) * std::pow(p.alpha, 3.0));
+    const double h = std::trunc(p.holes);
+    const double d = std::pow(h / g.rho, 1.0 / 3.0);
+    const double db = d * p.boundary_factor;
+    const double vol = d * d * d;
+    const double shell = db * db * db - vol;
+    g.side_m = d;
+    g.n_e0 = (int32_t)(p.N_e * p.e_ratio_start);
+    g.n_h = (int32_t)h + (int32_t)(h / vol * shell);
+}
+
 bool finite_all(const mcl_run_params& p)
 {
     const double v[] = {p.alpha, p.b, p.s, p.E_cb, p.E_loc_1, p.E_loc_2, p.D0, p.Retrap, p.k_b, p.D,
@@ -140,6 +155,8 @@ mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_i
     d.D = p.D;
     d.D0 = p.D0;
     d.duration = 0.0;
+    d.retrap_thr = p.Retrap >= 1.0 ? (1ull << 32)
+                                   : (p.Retrap > 0.0 ? (uint64_t)(p.Retrap * 0x1p32) : 0u);
     return d;
 }
 
@@ -638,13 +655,14 @@ int mcl_sync(void* ctx)
 }
 
 int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
-                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                const int32_t* obs_count, uint32_t options, uint64_t seed, int64_t max_steps,
                 double* ratio, int32_t* status, mcl_stats* stats)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return MCL_EINVAL;
     if (!runs || !obs_times || !obs_count || !ratio)
         return fail(c, MCL_EINVAL, "runs, obs_times, obs_count and ratio must not be null");
+    if (options & ~MCL_LAB_LEGACY) return fail(c, MCL_EINVAL, "unknown option bits 0x%x", options);
     if (n_runs <= 0) return fail(c, MCL_EINVAL, "n_runs must be > 0");
     if (max_steps <= 0) return fail(c, MCL_EINVAL, "max_steps must be > 0");
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
@@ -676,8 +694,12 @@ int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const dou
             return fail(c, MCL_EINVAL, "run %d: temperature must stay > 0 K", r);
         n_obs_all += k;
         max_obs = std::max(max_obs, k);
+        if (!(p.holes < 1e6) || !(p.N_e < 1e6) || !(p.boundary_factor < 100.0))
+            return fail(c, MCL_ERANGE, "run %d: lab clusters hold at most 8192 electrons and "
+                        "16384 holes", r);
         mcl_geometry g;
-        geometry(p, g);
+        if (options & MCL_LAB_LEGACY_GEOMETRY) legacy_geometry(p, g);
+        else geometry(p, g);
         if (g.n_e0 < 0 || g.n_h < 0 || g.n_e0 > 8192 || g.n_h > 16384 || p.N_e > 8192)
             return fail(c, MCL_ERANGE, "run %d: lab clusters hold at most 8192 electrons and "
                         "16384 holes", r);
@@ -760,7 +782,7 @@ int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const dou
     a.max_steps = max_steps;
     a.h_cap = h_cap;
     a.e_cap = e_cap;
-    a.exact_fill = exact_fill ? 1 : 0;
+    a.options = options;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);

@@ -30,6 +30,7 @@ struct RunDesc {
     double N_e;            // trap count: electron_ratio = n_e / N_e
     double D, D0;          // dose rate [Gy/s] and D0 of the filling rate
     double duration;       // [s] the loop runs while t <= duration
+    uint64_t retrap_thr;   // legacy retrapping: a new pair iff u < retrap_thr (2^32: always)
 };
 
 // Optional dump of one cluster's kernel state (tests / inspection).
@@ -103,7 +104,7 @@ struct LabArgs {
     uint64_t* work;             // [n_clusters][4] live electron-steps, fills, recombinations, steps
     int64_t max_steps;
     int32_t h_cap, e_cap;
-    int32_t exact_fill;         // 1: fills update every nearest hole exactly
+    uint32_t options;           // MCL_LAB_* bits
 };
 int lab_lds_bytes(int h_cap, int e_cap);
 

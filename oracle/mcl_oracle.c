@@ -214,6 +214,23 @@ int orc_run_geometry(const mcl_run_params* p, mcl_geometry* g)
     return 0;
 }
 
+/* initialize_box_bg (functions.py:51-80): side from int(holes), the boundary shell
+ * holds int(int(holes) / d^3 * ((bf d)^3 - d^3)) more holes. */
+int orc_run_geometry_legacy(const mcl_run_params* p, mcl_geometry* g)
+{
+    double rho = p->rho_prime * (3.0 / (4.0 * 0x1.921fb54442d18p+1) * pow(p->alpha, 3.0));
+    double h = trunc(p->holes);
+    double d = pow(h / rho, 1.0 / 3.0);
+    double bl = d * p->boundary_factor;
+    double box = d * d * d;
+    double box_b = bl * bl * bl;
+    g->rho = rho;
+    g->side_m = d;
+    g->n_e0 = (int32_t)(p->N_e * p->e_ratio_start);
+    g->n_h = (int32_t)h + (int32_t)((h / (d * d * d)) * (box_b - box));
+    return 0;
+}
+
 /* -------------------------------------------------------- cluster state --- */
 typedef struct {
     int n_e0, n_h;
@@ -840,12 +857,74 @@ static void lab_from_key(uint64_t key, float alpha_f, uint32_t* slot, float* w)
     *w = orc_expf(-(alpha_f * sqrtf(d2)));
 }
 
+#define LAB_EXACT_FILL 0x1u
+#define LAB_SHARED_CHANNEL 0x2u
+#define LAB_RETRAP 0x4u
+#define LAB_FIRST_ELECTRON 0x8u
+#define LAB_LEGACY_GEOMETRY 0x10u
+
+/* One electron-hole pair appended (Box.add_electron, engine.py:133-152); returns
+ * 0, or -2 when the cluster is full.  exact: every nearest hole exact after it. */
+static int lab_add_pair(lab_t* L, int* fills, int exact, int run, uint32_t c3e, uint32_t c3h,
+                        const uint32_t key[2], double side_nm, double side_b_nm, float alpha_f,
+                        uint32_t* remap)
+{
+    if (L->n_e >= L->e_cap) return -2;
+    if (L->n_hs >= L->h_cap) {
+        int base = 0;
+        for (int s = 0; s < L->n_hs; ++s) {
+            if (L->hx[s] == LAB_DEAD) { remap[s] = MCL_NO_HOLE; continue; }
+            remap[s] = (uint32_t)base;
+            L->hx[base] = L->hx[s]; L->hy[base] = L->hy[s]; L->hz[base] = L->hz[s];
+            ++base;
+        }
+        for (int e = 0; e < L->n_e; ++e) if (L->enn[e] != MCL_NO_HOLE) L->enn[e] = remap[L->enn[e]];
+        L->n_hs = base;
+        if (L->n_hs >= L->h_cap) return -2;
+    }
+    uint32_t oe[4], oh[4];
+    { uint32_t ctr[4] = {(uint32_t)*fills, 0xFFFFFFFEu, (uint32_t)run, c3e}; orc_philox4x32_10(ctr, key, oe); }
+    { uint32_t ctr[4] = {(uint32_t)*fills, 0xFFFFFFFEu, (uint32_t)run, c3h}; orc_philox4x32_10(ctr, key, oh); }
+    const float ex = (float)(((double)oe[0] * 0x1p-32) * side_nm), ey = (float)(((double)oe[1] * 0x1p-32) * side_nm),
+                ez = (float)(((double)oe[2] * 0x1p-32) * side_nm);
+    const int hs = L->n_hs;
+    L->hx[hs] = (float)(((double)oh[0] * 0x1p-32) * side_b_nm);
+    L->hy[hs] = (float)(((double)oh[1] * 0x1p-32) * side_b_nm);
+    L->hz[hs] = (float)(((double)oh[2] * 0x1p-32) * side_b_nm);
+    uint32_t slot;
+    float w;
+    lab_from_key(lab_nearest(L, exact ? hs + 1 : hs, ex, ey, ez), alpha_f, &slot, &w);
+    if (exact) {
+        for (int e = 0; e < L->n_e; ++e) {
+            float dx = L->hx[hs] - L->ex[e], dy = L->hy[hs] - L->ey[e], dz = L->hz[hs] - L->ez[e];
+            float d2n = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float d2o = INFINITY;
+            const uint32_t s0 = L->enn[e];
+            if (s0 != MCL_NO_HOLE) {
+                float ox = L->hx[s0] - L->ex[e], oy = L->hy[s0] - L->ey[e], oz = L->hz[s0] - L->ez[e];
+                d2o = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+            }
+            if (d2n < d2o) { L->enn[e] = (uint32_t)hs; L->ew[e] = orc_expf(-(alpha_f * sqrtf(d2n))); }
+        }
+    }
+    L->ex[L->n_e] = ex; L->ey[L->n_e] = ey; L->ez[L->n_e] = ez; L->ew[L->n_e] = w; L->enn[L->n_e] = slot;
+    L->n_e += 1; L->n_hs += 1; *fills += 1;
+    return 0;
+}
+
 static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const uint32_t key[2],
-                               const double* obs, int n_obs, int exact_fill, int64_t max_steps,
+                               const double* obs, int n_obs, uint32_t options, int64_t max_steps,
                                int h_cap, int e_cap, double* out)
 {
     mcl_geometry g;
-    orc_run_geometry(p, &g);
+    if (options & LAB_LEGACY_GEOMETRY) orc_run_geometry_legacy(p, &g);
+    else orc_run_geometry(p, &g);
+    const int exact_fill = (options & LAB_EXACT_FILL) != 0;
+    const int shared_ch = (options & LAB_SHARED_CHANNEL) != 0;
+    const int first_e = (options & LAB_FIRST_ELECTRON) != 0;
+    const int retrap = (options & LAB_RETRAP) != 0;
+    const uint64_t retrap_thr = p->Retrap >= 1.0 ? (1ull << 32)
+                              : (p->Retrap > 0.0 ? (uint64_t)(p->Retrap * 0x1p32) : 0u);
     const double side_nm = g.side_m * 1e9, side_b_nm = (g.side_m * p->boundary_factor) * 1e9;
     lab_t L = {0};
     L.h_cap = h_cap; L.e_cap = e_cap;
@@ -897,8 +976,12 @@ static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const 
         const double B2 = b2_is_b1 ? B1 : p->b * orc_exp(-(p->E_loc_2 / kT));
         const int rows = (L.n_e + 63) / 64;
         double K = 0.0;
+        /* the shared draw: one word per step for every electron (functions.py:125) */
+        const int ch_step = (two_ch && shared_ch)
+            ? (uint64_t)philox_word((uint32_t)(i >> 2), 0xFFFFFFFDu, (uint32_t)run, c3c, key, (int)(i & 3)) < ret_thr
+            : always2;
 #define LAB_RATE(pp) ((pp) >= L.n_e || L.enn[(pp)] == MCL_NO_HOLE ? 0.0 : \
-        fma(((two_ch ? (uint64_t)philox_word((uint32_t)(i >> 2), (uint32_t)(pp), (uint32_t)run, c3c, key, (int)(i & 3)) < ret_thr : always2) ? B2 : B1), \
+        fma(((two_ch && !shared_ch ? (uint64_t)philox_word((uint32_t)(i >> 2), (uint32_t)(pp), (uint32_t)run, c3c, key, (int)(i & 3)) < ret_thr : ch_step) ? B2 : B1), \
             (double)L.ew[(pp)], A))
         for (int r = 0; r < rows; ++r) {
             for (int l = 0; l < 64; ++l) v[l] = LAB_RATE(r * 64 + l);
@@ -915,50 +998,16 @@ static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const 
         const double dt = fmin(dt_rec, dt_fill);
         t += dt;
         if (dt == dt_fill) {
-            if (L.n_e >= e_cap) { status = -2; break; }
-            if (L.n_hs >= h_cap) {
-                int base = 0;
-                for (int s = 0; s < L.n_hs; ++s) {
-                    if (L.hx[s] == LAB_DEAD) { remap[s] = MCL_NO_HOLE; continue; }
-                    remap[s] = (uint32_t)base;
-                    L.hx[base] = L.hx[s]; L.hy[base] = L.hy[s]; L.hz[base] = L.hz[s];
-                    ++base;
-                }
-                for (int e = 0; e < L.n_e; ++e) if (L.enn[e] != MCL_NO_HOLE) L.enn[e] = remap[L.enn[e]];
-                L.n_hs = base;
-                if (L.n_hs >= h_cap) { status = -2; break; }
+            if (lab_add_pair(&L, &fills, exact_fill, run, c3e, c3h, key, side_nm, side_b_nm, alpha_f, remap)) {
+                status = -2;
+                break;
             }
-            uint32_t oe[4], oh[4];
-            { uint32_t ctr[4] = {(uint32_t)fills, 0xFFFFFFFEu, (uint32_t)run, c3e}; orc_philox4x32_10(ctr, key, oe); }
-            { uint32_t ctr[4] = {(uint32_t)fills, 0xFFFFFFFEu, (uint32_t)run, c3h}; orc_philox4x32_10(ctr, key, oh); }
-            const float ex = (float)(((double)oe[0] * 0x1p-32) * side_nm), ey = (float)(((double)oe[1] * 0x1p-32) * side_nm),
-                        ez = (float)(((double)oe[2] * 0x1p-32) * side_nm);
-            const int hs = L.n_hs;
-            L.hx[hs] = (float)(((double)oh[0] * 0x1p-32) * side_b_nm);
-            L.hy[hs] = (float)(((double)oh[1] * 0x1p-32) * side_b_nm);
-            L.hz[hs] = (float)(((double)oh[2] * 0x1p-32) * side_b_nm);
-            uint32_t slot;
-            float w;
-            lab_from_key(lab_nearest(&L, exact_fill ? hs + 1 : hs, ex, ey, ez), alpha_f, &slot, &w);
-            if (exact_fill) {
-                for (int e = 0; e < L.n_e; ++e) {
-                    float dx = L.hx[hs] - L.ex[e], dy = L.hy[hs] - L.ey[e], dz = L.hz[hs] - L.ez[e];
-                    float d2n = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    float d2o = INFINITY;
-                    const uint32_t s0 = L.enn[e];
-                    if (s0 != MCL_NO_HOLE) {
-                        float ox = L.hx[s0] - L.ex[e], oy = L.hy[s0] - L.ey[e], oz = L.hz[s0] - L.ez[e];
-                        d2o = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
-                    }
-                    if (d2n < d2o) { L.enn[e] = (uint32_t)hs; L.ew[e] = orc_expf(-(alpha_f * sqrtf(d2n))); }
-                }
-            }
-            L.ex[L.n_e] = ex; L.ey[L.n_e] = ey; L.ez[L.n_e] = ez; L.ew[L.n_e] = w; L.enn[L.n_e] = slot;
-            L.n_e += 1; L.n_hs += 1; fills += 1;
         } else if (dt == dt_rec) {
             const double target = (double)sb[1] * 0x1p-32 * K;
             int pw = -1;
             double carry = 0.0;
+            if (first_e)      /* recomber as the legacy loops call it: the oldest electron */
+                for (int s2 = 0; s2 < L.n_e && pw < 0; ++s2) if (L.enn[s2] != MCL_NO_HOLE) pw = s2;
             for (int r = 0; r < rows && pw < 0; ++r) {
                 const double c2 = carry + rtot[r];
                 if (c2 > target) {
@@ -982,6 +1031,12 @@ static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const 
             for (int e = 0; e < L.n_e; ++e)
                 if (L.enn[e] == hw)
                     lab_from_key(lab_nearest(&L, L.n_hs, L.ex[e], L.ey[e], L.ez[e]), alpha_f, &L.enn[e], &L.ew[e]);
+            /* legacy retrapping (functions.py:328-331): a new pair, exact nearest holes */
+            if (retrap && (uint64_t)sb[3] < retrap_thr &&
+                lab_add_pair(&L, &fills, 1, run, c3e, c3h, key, side_nm, side_b_nm, alpha_f, remap)) {
+                status = -2;
+                break;
+            }
         }
         ++i;
         while (k_obs < n_obs && t >= obs[k_obs]) { out[k_obs] = (double)L.n_e / p->N_e; ++k_obs; }
@@ -992,7 +1047,7 @@ static int32_t run_cluster_lab(const mcl_run_params* p, int run, int sim, const 
 }
 
 int orc_run_lab(const mcl_run_params* runs, int32_t n_runs, const double* obs_times,
-                const int32_t* obs_count, int32_t exact_fill, uint64_t seed, int64_t max_steps,
+                const int32_t* obs_count, uint32_t options, uint64_t seed, int64_t max_steps,
                 int32_t n_threads, double* ratio, int32_t* status)
 {
     if (!runs || n_runs <= 0 || !obs_times || !obs_count || !ratio) return MCL_EINVAL;
@@ -1007,7 +1062,8 @@ int orc_run_lab(const mcl_run_params* runs, int32_t n_runs, const double* obs_ti
         total += runs[r].sims;
         if (runs[r].sims > 0) {
             mcl_geometry g;
-            orc_run_geometry(&runs[r], &g);
+            if (options & LAB_LEGACY_GEOMETRY) orc_run_geometry_legacy(&runs[r], &g);
+            else orc_run_geometry(&runs[r], &g);
             int ne = g.n_e0 > (int)runs[r].N_e ? g.n_e0 : (int)runs[r].N_e;
             if (ne > max_e) max_e = ne;
             if (g.n_h > max_h) max_h = g.n_h;
@@ -1030,7 +1086,7 @@ int orc_run_lab(const mcl_run_params* runs, int32_t n_runs, const double* obs_ti
         memset(out, 0, sizeof(double) * (size_t)max_obs);
         const int r = c_run[c];
         const int32_t st = run_cluster_lab(&runs[r], r, c_sim[c], key, obs_times + offs[r],
-                                           obs_count[r], exact_fill, max_steps, h_cap, e_cap, out);
+                                           obs_count[r], options, max_steps, h_cap, e_cap, out);
         if (status) status[c] = st;
     }
     free(c_run); free(c_sim); free(offs);

@@ -93,10 +93,12 @@ def run_event(runs: Sequence[RunParams], duration, max_steps: int, seed: int = D
 
 
 def run_lab(runs: Sequence[RunParams], observations, exact_fill: bool = False,
-            seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7, threads: int = 0):
-    """Replay of the lab kernel: (ratio [clusters, max_obs], status)."""
+            seed: int = DEFAULT_SEED, max_steps: int = 10 ** 7, threads: int = 0,
+            options: int = 0):
+    """Replay of the lab kernel: (ratio [clusters, max_obs], status).  options: the
+    MCL_LAB_* bits (exact_fill sets MCL_LAB_EXACT_FILL)."""
     L = lib()
-    L.orc_run_lab.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+    L.orc_run_lab.argtypes = [C.POINTER(RunParams), C.c_int32, C.c_void_p, C.c_void_p, C.c_uint32,
                               C.c_uint64, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
     L.orc_run_lab.restype = C.c_int
     arr = run_array(runs)
@@ -105,7 +107,8 @@ def run_lab(runs: Sequence[RunParams], observations, exact_fill: bool = False,
     total = sum(int(r.sims) for r in runs)
     ratio = np.zeros((total, int(counts.max())), dtype=np.float64)
     status = np.zeros(total, dtype=np.int32)
-    st = L.orc_run_lab(arr, len(runs), obs.ctypes.data, counts.ctypes.data, 1 if exact_fill else 0,
+    opts = int(options) | (1 if exact_fill else 0)
+    st = L.orc_run_lab(arr, len(runs), obs.ctypes.data, counts.ctypes.data, opts,
                        C.c_uint64(seed), int(max_steps), int(threads), ratio.ctypes.data,
                        status.ctypes.data)
     if st != 0:
@@ -146,6 +149,15 @@ def exp64(x: float) -> float:
 def geometry(run: RunParams) -> Geometry:
     g = Geometry()
     lib().orc_run_geometry(C.byref(run), C.byref(g))
+    return g
+
+
+def geometry_legacy(run: RunParams) -> Geometry:
+    """The legacy stack's box (functions.py:51-80)."""
+    L = lib()
+    L.orc_run_geometry_legacy.argtypes = [C.POINTER(RunParams), C.POINTER(Geometry)]
+    g = Geometry()
+    L.orc_run_geometry_legacy(C.byref(run), C.byref(g))
     return g
 
 
