@@ -188,7 +188,7 @@ __device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x,
     while (fb != 0) {
         const int l = __builtin_ctzll(fb);
         fb &= fb - 1;
-        const float fx = __shfl(x, l), fy = __shfl(y, l), fz = __shfl(z, l);
+        const float fx = lane_value(x, l), fy = lane_value(y, l), fz = lane_value(z, l);
         const uint64_t k = full_scan_wave(g, fx, fy, fz, lane, evals);
         if (lane == l) key = k;
     }
@@ -316,8 +316,8 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
         const uint32_t qq = lane < nq ? rq[lane] : 0u;
         if (lane < nq) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
         for (int i = 0; i < nq; ++i) {
-            const uint32_t q = (uint32_t)__shfl((int)qq, i);
-            const float x = __shfl(qx, i), y = __shfl(qy, i), z = __shfl(qz, i);
+            const uint32_t q = lane_value(qq, i);
+            const float x = lane_value(qx, i), y = lane_value(qy, i), z = lane_value(qz, i);
             bool ok = true;
             uint64_t key = block_search_wave(grid, x, y, z, lane, ok, dist_evals);
             if (!ok) key = full_scan_wave(grid, x, y, z, lane, dist_evals);

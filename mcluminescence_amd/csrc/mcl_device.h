@@ -206,16 +206,58 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v)
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Value of lane l (l wave-uniform) in every lane: v_readlane, no LDS round trip.
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), l));
+}
+
+// DPP controls (gfx9 encoding): quad permutations and the row mirrors.
+constexpr int kDppQuad1032 = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuad2301 = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xF, 0xF, false);
+}
+
+// Minimum over the 64 lanes (all active), in every lane: four DPP steps give the
+// minimum of each 16-lane row, four readlanes and scalar mins combine the rows.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+    uint32_t u = dpp_u32<kDppQuad1032>(v);
+    v = u < v ? u : v;
+    u = dpp_u32<kDppQuad2301>(v);
+    v = u < v ? u : v;
+    u = dpp_u32<kDppRowHalfMirror>(v);
+    v = u < v ? u : v;
+    u = dpp_u32<kDppRowMirror>(v);
+    v = u < v ? u : v;
+    const uint32_t r0 = lane_value(v, 0), r1 = lane_value(v, 16);
+    const uint32_t r2 = lane_value(v, 32), r3 = lane_value(v, 48);
+    const uint32_t a = r0 < r1 ? r0 : r1, b = r2 < r3 ? r2 : r3;
+    return a < b ? a : b;
+}
+
+// Minimum of a 64-bit key over the 64 lanes (all active): the high words first;
+// the low word of the single lane holding that minimum, or the minimum low word
+// among the tied lanes.
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off);
-        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), off);
-        const uint64_t o = ((uint64_t)hi << 32) | lo;
-        v = o < v ? o : v;
-    }
-    return v;
+    const uint32_t hi = (uint32_t)(v >> 32), lo = (uint32_t)v;
+    const uint32_t mh = wave_min_u32(hi);
+    const uint64_t tied = __ballot(hi == mh);
+    uint32_t ml;
+    if ((tied & (tied - 1)) == 0) ml = lane_value(lo, (int)__builtin_ctzll(tied));
+    else ml = wave_min_u32(hi == mh ? lo : 0xFFFFFFFFu);
+    return ((uint64_t)mh << 32) | ml;
 }
 
 __device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
