@@ -118,47 +118,97 @@ __device__ __forceinline__ uint64_t block_search_lane(const Grid& g, float x, fl
     return key;
 }
 
-// Wave-cooperative search for one electron (x, y, z wave-uniform).  The 3x3x3
-// block is at most 9 contiguous slot ranges (one per (z, y) row of cells); lane
-// l takes range l / 7 at offsets l % 7, l % 7 + 7, ... so every lane finds its
-// slots from two cell starts, with no cross-lane bookkeeping, then a wave min.
-// Ranges longer than 7 holes cost one more pass each 7.
-__device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, float y, float z,
-                                                      int lane, bool& conclusive, uint64_t& evals)
-{
-    constexpr int kPer = 7;                          // lanes per range (9 x 7 = 63 lanes)
-    const int G = g.G;
-    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
-              cz = cell_of(z, g.inv_cell, G);
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
-    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
-    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
-    const int ny = y1 - y0 + 1;
-    const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
-    const int j = lane / kPer, o = lane - j * kPer;
-    int s = 0, e = 0;
-    if (j < nr) {
-        const int zz = z0 + j / ny, yy = y0 + j % ny;
-        const int base = (zz * G + yy) * G;
-        s = (int)g.cstart[base + x0];
-        e = (int)g.cstart[base + x1 + 1];
+// 3x3x3 cell block around (x, y, z): cell bounds, clamped to the grid.
+struct Block {
+    int x0, x1, y0, y1, z0, z1;
+    __device__ __forceinline__ Block(const Grid& g, float x, float y, float z)
+    {
+        const int G = g.G;
+        const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
+                  cz = cell_of(z, g.inv_cell, G);
+        x0 = max(cx - 1, 0);
+        x1 = min(cx + 1, G - 1);
+        y0 = max(cy - 1, 0);
+        y1 = min(cy + 1, G - 1);
+        z0 = max(cz - 1, 0);
+        z1 = min(cz + 1, G - 1);
     }
-    if (o == 0) evals += (uint64_t)(e - s);          // summed over lanes at the end
-    uint64_t key = ~0ull;
-    for (int slot = s + o;; slot += kPer) {
-        const bool v = slot < e;
-        if (ballot(v) == 0) break;
-        if (v) {
-            const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
-            key = k < key ? k : key;
+    // Is the block minimum d2 provably the nearest live hole of (x, y, z)?
+    __device__ __forceinline__ bool conclusive(const Grid& g, float x, float y, float z,
+                                               float best_d2) const
+    {
+        const int G = g.G;
+        const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 &&
+                           z1 == G - 1;
+        const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
+                                fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
+        return whole || block_conclusive(best_d2, gap, g.cell);
+    }
+};
+
+// Wave-cooperative 3x3x3 block searches for up to kSearchBatch electrons at once
+// (electron b's coordinates in lane b).  A block is at most 9 contiguous slot
+// ranges (one per (z, y) row of cells); lane l takes range l / 7 at offsets l % 7,
+// l % 7 + 7, ... (9 x 7 = 63 lanes), finding its slots from two cell starts.  The
+// electrons' loads, first two passes and wave minima are independent, so their
+// LDS round trips overlap; longer ranges take further passes one electron at a
+// time.  Returns electron b's block minimum key in keys[b] (wave-uniform).
+constexpr int kSearchBatch = 4;
+__device__ __forceinline__ void block_search_batch(const Grid& g, int nb, float qx, float qy,
+                                                   float qz, int lane,
+                                                   uint64_t (&keys)[kSearchBatch],
+                                                   uint64_t& evals)
+{
+    constexpr int kPer = 7;
+    const int G = g.G;
+    const int j = lane / kPer, o = lane - j * kPer;
+    int s[kSearchBatch], e[kSearchBatch];
+    float bx[kSearchBatch], by[kSearchBatch], bz[kSearchBatch];
+#pragma unroll
+    for (int b = 0; b < kSearchBatch; ++b) {
+        bx[b] = lane_value(qx, b);
+        by[b] = lane_value(qy, b);
+        bz[b] = lane_value(qz, b);
+        s[b] = 0;
+        e[b] = 0;
+        if (b < nb) {
+            const Block k(g, bx[b], by[b], bz[b]);
+            const int ny = k.y1 - k.y0 + 1;
+            if (j < (k.z1 - k.z0 + 1) * ny) {
+                const int base = ((k.z0 + j / ny) * G + k.y0 + j % ny) * G;
+                s[b] = (int)g.cstart[base + k.x0];
+                e[b] = (int)g.cstart[base + k.x1 + 1];
+            }
+            if (o == 0) evals += (uint64_t)(e[b] - s[b]);   // summed over lanes at the end
         }
     }
-    key = wave_min_u64(key);
-    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
-    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
-                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
-    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
-    return key;
+    uint64_t key[kSearchBatch];
+#pragma unroll
+    for (int b = 0; b < kSearchBatch; ++b) {
+        key[b] = ~0ull;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            const int slot = s[b] + o + pass * kPer;
+            if (slot < e[b]) {
+                const uint64_t k = key_of(g.hole[slot], bx[b], by[b], bz[b], slot);
+                key[b] = k < key[b] ? k : key[b];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kSearchBatch; ++b) {
+        if (b >= nb) break;
+        for (int slot = s[b] + o + 2 * kPer;; slot += kPer) {
+            const bool v = slot < e[b];
+            if (ballot(v) == 0) break;
+            if (v) {
+                const uint64_t k = key_of(g.hole[slot], bx[b], by[b], bz[b], slot);
+                key[b] = k < key[b] ? k : key[b];
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kSearchBatch; ++b) keys[b] = b < nb ? wave_min_u64(key[b]) : ~0ull;
 }
 
 // The whole wave scans every hole for one electron (x, y, z wave-uniform).
@@ -304,29 +354,45 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
 }
 
 // New nearest holes for the queued electrons rq[0..nq) ((slot << 16) | index):
-// one electron at a time with the whole wave while the queue is short (at most
-// wave_search_max), lane-parallel otherwise; writes est and ew of their slots.
+// wave-cooperative searches, kSearchBatch electrons at a time, while the queue
+// is short (at most wave_search_max), lane-parallel otherwise; writes est and ew
+// of their slots.
 __device__ __forceinline__ void search_queue_nn(const Grid& grid, const ElectronGen& egen,
                                                 uint32_t* est, float* ew, const uint32_t* rq,
                                                 int nq, int wave_search_max, float alpha_f,
                                                 int lane, uint64_t& dist_evals)
 {
     if (nq <= wave_search_max) {
-        float qx = 0.0f, qy = 0.0f, qz = 0.0f;
-        const uint32_t qq = lane < nq ? rq[lane] : 0u;
-        if (lane < nq) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
-        for (int i = 0; i < nq; ++i) {
-            const uint32_t q = lane_value(qq, i);
-            const float x = lane_value(qx, i), y = lane_value(qy, i), z = lane_value(qz, i);
+        for (int i0 = 0; i0 < nq; i0 += kSearchBatch) {
+            const int nb = min(kSearchBatch, nq - i0);
+            const bool mine = lane < nb;               // lane b carries electron b
+            const uint32_t qq = mine ? rq[i0 + lane] : 0u;
+            float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+            if (mine) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
+            uint64_t keys[kSearchBatch];
+            block_search_batch(grid, nb, qx, qy, qz, lane, keys, dist_evals);
+            uint64_t key = keys[0];
+#pragma unroll
+            for (int b = 1; b < kSearchBatch; ++b) key = lane == b ? keys[b] : key;
             bool ok = true;
-            uint64_t key = block_search_wave(grid, x, y, z, lane, ok, dist_evals);
-            if (!ok) key = full_scan_wave(grid, x, y, z, lane, dist_evals);
-            uint32_t slot;
-            float w;
-            nn_from_key(key, alpha_f, slot, w);
-            if (lane == 0) {
-                est[q >> 16] = (slot << 16) | (q & 0xFFFFu);
-                ew[q >> 16] = w;
+            if (mine) {
+                const Block k(grid, qx, qy, qz);
+                ok = k.conclusive(grid, qx, qy, qz, __uint_as_float((uint32_t)(key >> 32)));
+            }
+            uint64_t fb = ballot(mine && !ok);
+            while (fb != 0) {
+                const int l = __builtin_ctzll(fb);
+                fb &= fb - 1;
+                const uint64_t k = full_scan_wave(grid, lane_value(qx, l), lane_value(qy, l),
+                                                  lane_value(qz, l), lane, dist_evals);
+                if (lane == l) key = k;
+            }
+            if (mine) {
+                uint32_t slot;
+                float w;
+                nn_from_key(key, alpha_f, slot, w);
+                est[qq >> 16] = (slot << 16) | (qq & 0xFFFFu);
+                ew[qq >> 16] = w;
             }
         }
         return;

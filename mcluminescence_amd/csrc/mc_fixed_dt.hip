@@ -10,7 +10,8 @@
 //   cstart     [g_cap]   first sorted slot of each cell, cstart[G^3] = n_h
 //   est        [e_cap]   compacted live electrons: (nearest hole slot << 16) | electron index
 //   ew         [e_cap]   compacted w = exp(-alpha r_nn), float32
-//   ethr       [e_cap]   fire threshold of a candidate in the current step (conflict retries)
+//   ethr       [e_cap]   fire threshold of a candidate in the current step (conflict
+//                        retries); the draw of an electron queued for a new search
 //   rq         [e_cap]   electrons whose nearest hole must be searched again: (slot << 16) | e
 //
 // Electron coordinates are not stored: they are a pure function of the
@@ -248,7 +249,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         // (its test waits for it); one whose draw fails even against the old w stays
         // as it is; every other live electron takes its fire test now.  Deferring
         // the search to here changes nothing: no hole disappears in between.
-        uint64_t cand = 0, stale = 0;
+        uint64_t cand = 0, stale = 0, stale_ch = 0;
         int nq = 0;
         for (int r = 0; r < rows; r += 2) {
             const int p0 = r * kWave + lane, p1 = p0 + kWave;
@@ -272,10 +273,11 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             }
             const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
             const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
+            // a queued electron keeps its draw in its (still unused) threshold slot
             if (!s0) fire_tail(st0, o0, p0, u0, x0, r, cand);
-            else s0 = may_fire(u0, x0);
+            else if ((s0 = may_fire(u0, x0))) ethr[p0] = u0;
             if (!s1) fire_tail(st1, o1, p1, u1, x1, r + 1, cand);
-            else s1 = may_fire(u1, x1);
+            else if ((s1 = may_fire(u1, x1))) ethr[p1] = u1;
             const uint64_t m0 = ballot(s0);
             if (s0) rq[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
             nq += __popcll(m0);
@@ -283,6 +285,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             if (s1) rq[nq + lanes_below(m1)] = ((uint32_t)p1 << 16) | o1;
             nq += __popcll(m1);
             stale |= ((uint64_t)s0 << r) | ((uint64_t)s1 << (r + 1));
+            stale_ch |= ((uint64_t)ch0 << r) | ((uint64_t)ch1 << (r + 1));
         }
         if (nq > 0) {
             __syncthreads();
@@ -294,13 +297,11 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                 m &= m - 1;
                 const int p = r * kWave + lane;
                 if ((stale >> r) & 1ull) {
+                    // the draw and channel of pass A, the new w
                     const uint32_t st = est[p];
-                    const uint32_t o = st & 0xFFFFu;
-                    const uint32_t u = pick_word(philox10(blk, o, prun, c3f, k0, k1), word);
-                    bool ch = always2;
-                    if (two_ch)
-                        ch = (uint64_t)pick_word(philox10(blk, o, prun, c3c, k0, k1), word) < ret_thr;
-                    fire_tail(st, o, p, u, __builtin_fmaf(ch ? b2dt : b1dt, ew[p], xA), r, cand);
+                    const bool ch = (stale_ch >> r) & 1ull;
+                    fire_tail(st, st & 0xFFFFu, p, ethr[p], __builtin_fmaf(ch ? b2dt : b1dt, ew[p], xA),
+                              r, cand);
                 }
             }
         }
