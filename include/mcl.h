@@ -112,8 +112,9 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *   "persist_frac"    persistent workgroups per size class, as a fraction of
  *                     what fits on the device at once (default 1).
  *   "wave_search_max" electrons needing a new nearest hole in one step are
- *                     searched one at a time by the whole wave up to this
- *                     many, lane-parallel beyond (default 4, at most 64).
+ *                     searched by the whole wave, up to four at a time, up
+ *                     to this many, lane-parallel beyond (default 32, at
+ *                     most 64).
  *   "lookahead_max"   with at most this many live electrons in a cluster, the
  *                     wave tests the next 64 steps at once and skips those in
  *                     which no electron can fire (default 8; 0 turns it off).

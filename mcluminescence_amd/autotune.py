@@ -4,8 +4,9 @@ The kernels' results do not depend on the tuning knobs (mcl_tune_set):
 
   holes_per_cell   density of each cluster's nearest-hole cell grid (search work
                    per lookup against the rate of inconclusive 3x3x3 searches)
-  wave_search_max  queue length up to which a step's nearest-hole searches run one
-                   electron per wave rather than lane-parallel
+  wave_search_max  queue length up to which a step's nearest-hole searches run
+                   wave-cooperatively (four electrons at a time) rather than
+                   lane-parallel
   lookahead_max    live electrons up to which a cluster tests 64 steps at once and
                    skips the quiet ones
   persist_frac     persistent workgroups per size class, as a fraction of what
@@ -29,10 +30,10 @@ from typing import Dict, Iterable, List, Optional, Sequence
 from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geometry
 
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
-DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 4.0,
+DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 32.0,
             "lookahead_max": 8.0}
 SPACE = {"holes_per_cell": [2.0, 3.0, 4.0], "persist_frac": [1.0, 2.0],
-         "wave_search_max": [2.0, 4.0, 8.0], "lookahead_max": [0.0, 8.0, 32.0]}
+         "wave_search_max": [8.0, 32.0, 64.0], "lookahead_max": [0.0, 8.0, 32.0]}
 
 
 def signature(runs: Sequence[RunParams]) -> str:

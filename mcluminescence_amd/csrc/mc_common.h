@@ -146,6 +146,49 @@ struct Block {
     }
 };
 
+// Wave-cooperative search for one electron (x, y, z wave-uniform).  The 3x3x3
+// block is at most 9 contiguous slot ranges (one per (z, y) row of cells); lane
+// l takes range l / 7 at offsets l % 7, l % 7 + 7, ... so every lane finds its
+// slots from two cell starts, with no cross-lane bookkeeping, then a wave min.
+// Ranges longer than 7 holes cost one more pass each 7.
+__device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, float y, float z,
+                                                      int lane, bool& conclusive, uint64_t& evals)
+{
+    constexpr int kPer = 7;                          // lanes per range (9 x 7 = 63 lanes)
+    const int G = g.G;
+    const int cx = cell_of(x, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G),
+              cz = cell_of(z, g.inv_cell, G);
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
+    const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
+    const int ny = y1 - y0 + 1;
+    const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
+    const int j = lane / kPer, o = lane - j * kPer;
+    int s = 0, e = 0;
+    if (j < nr) {
+        const int zz = z0 + j / ny, yy = y0 + j % ny;
+        const int base = (zz * G + yy) * G;
+        s = (int)g.cstart[base + x0];
+        e = (int)g.cstart[base + x1 + 1];
+    }
+    if (o == 0) evals += (uint64_t)(e - s);          // summed over lanes at the end
+    uint64_t key = ~0ull;
+    for (int slot = s + o;; slot += kPer) {
+        const bool v = slot < e;
+        if (ballot(v) == 0) break;
+        if (v) {
+            const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
+            key = k < key ? k : key;
+        }
+    }
+    key = wave_min_u64(key);
+    const bool whole = x0 == 0 && y0 == 0 && z0 == 0 && x1 == G - 1 && y1 == G - 1 && z1 == G - 1;
+    const float gap = fminf(face_gap(x, x0, x1, G, g.cell),
+                            fminf(face_gap(y, y0, y1, G, g.cell), face_gap(z, z0, z1, G, g.cell)));
+    conclusive = whole || block_conclusive(__uint_as_float((uint32_t)(key >> 32)), gap, g.cell);
+    return key;
+}
+
 // Wave-cooperative 3x3x3 block searches for up to kSearchBatch electrons at once
 // (electron b's coordinates in lane b).  A block is at most 9 contiguous slot
 // ranges (one per (z, y) row of cells); lane l takes range l / 7 at offsets l % 7,
@@ -369,17 +412,27 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
             const uint32_t qq = mine ? rq[i0 + lane] : 0u;
             float qx = 0.0f, qy = 0.0f, qz = 0.0f;
             if (mine) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
-            uint64_t keys[kSearchBatch];
-            block_search_batch(grid, nb, qx, qy, qz, lane, keys, dist_evals);
-            uint64_t key = keys[0];
+            uint64_t key;
+            uint64_t fb;
+            if (nb == 1) {
+                // one electron: the single search ends after the passes its ranges need
+                bool ok = true;
+                key = block_search_wave(grid, lane_value(qx, 0), lane_value(qy, 0),
+                                        lane_value(qz, 0), lane, ok, dist_evals);
+                fb = ok ? 0ull : 1ull;
+            } else {
+                uint64_t keys[kSearchBatch];
+                block_search_batch(grid, nb, qx, qy, qz, lane, keys, dist_evals);
+                key = keys[0];
 #pragma unroll
-            for (int b = 1; b < kSearchBatch; ++b) key = lane == b ? keys[b] : key;
-            bool ok = true;
-            if (mine) {
-                const Block k(grid, qx, qy, qz);
-                ok = k.conclusive(grid, qx, qy, qz, __uint_as_float((uint32_t)(key >> 32)));
+                for (int b = 1; b < kSearchBatch; ++b) key = lane == b ? keys[b] : key;
+                bool ok = true;
+                if (mine) {
+                    const Block k(grid, qx, qy, qz);
+                    ok = k.conclusive(grid, qx, qy, qz, __uint_as_float((uint32_t)(key >> 32)));
+                }
+                fb = ballot(mine && !ok);
             }
-            uint64_t fb = ballot(mine && !ok);
             while (fb != 0) {
                 const int l = __builtin_ctzll(fb);
                 fb &= fb - 1;

@@ -20,7 +20,7 @@ struct Ctx {
     int n_cu = 0;
     double holes_per_cell = 2.0;   // hole-grid density target (tunable)
     double persist_frac = 1.0;     // persistent workgroups per class / device capacity
-    double wave_search_max = 4.0;  // queue length searched one electron per wave
+    double wave_search_max = 32.0; // queue length searched by the whole wave
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
     std::string err;
 };
