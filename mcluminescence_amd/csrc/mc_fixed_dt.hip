@@ -60,6 +60,12 @@
 
 namespace mcl {
 
+constexpr int kCachePairs = 4;    // row pairs whose Philox blocks stay in registers
+template <int V>
+struct CacheSlot {
+    static constexpr int value = V;
+};
+
 // Simulate cluster c from its geometry to its last step, all state in lds_raw.
 __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                             unsigned char* lds_raw)
@@ -217,6 +223,15 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         b2dt = sc.b2dt;
         tA = sc.tA;
     }
+    // Philox blocks of the first 2 kCachePairs rows, kept in registers: steps
+    // n .. n+3 share the counter (n >> 2, e, run, sim|tag), so a block computed in
+    // one of them serves the other three.  cache_pairs marks the row pairs whose
+    // blocks are those of block cache_blk for the electrons now in their slots.
+    u32x4 cfa[kCachePairs], cfb[kCachePairs];   // fire blocks of rows 2q, 2q + 1
+    uint32_t cch[kCachePairs];                   // channel bits: row 2q word k at bit k,
+                                                 // row 2q + 1 at bit 4 + k
+    uint32_t cache_pairs = 0u;
+    int cache_blk = -1;
     int n = 0;
     while (n < step_limit && n_live > 0) {
         if (n_live <= a.lookahead_max) {
@@ -251,7 +266,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         // the search to here changes nothing: no hole disappears in between.
         uint64_t cand = 0, stale = 0, stale_ch = 0;
         int nq = 0;
-        for (int r = 0; r < rows; r += 2) {
+        if ((int)blk != cache_blk) {
+            cache_blk = (int)blk;
+            cache_pairs = 0u;
+        }
+        auto pass_a_pair = [&](int r, auto slot_tag) {
+            constexpr int q = decltype(slot_tag)::value;   // register cache slot, -1: none
             const int p0 = r * kWave + lane, p1 = p0 + kWave;
             const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
             const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
@@ -263,13 +283,42 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             // there (early in a heating ramp) skip their draws.
             if (ballot(__builtin_fmaf(bmax, w0, xA) >= 0x1p-32f ||
                        __builtin_fmaf(bmax, w1, xA) >= 0x1p-32f) == 0)
-                continue;
-            const uint32_t u0 = pick_word(philox10(blk, o0, prun, c3f, k0, k1), word);
-            const uint32_t u1 = pick_word(philox10(blk, o1, prun, c3f, k0, k1), word);
+                return;
+            uint32_t u0, u1;
             bool ch0 = always2, ch1 = always2;
-            if (two_ch) {
-                ch0 = (uint64_t)pick_word(philox10(blk, o0, prun, c3c, k0, k1), word) < ret_thr;
-                ch1 = (uint64_t)pick_word(philox10(blk, o1, prun, c3c, k0, k1), word) < ret_thr;
+            bool hit = false;
+            if constexpr (q >= 0) hit = ((cache_pairs >> q) & 1u) != 0u;
+            if (hit) {
+                if constexpr (q >= 0) {
+                    u0 = pick_word(cfa[q], word);
+                    u1 = pick_word(cfb[q], word);
+                    if (two_ch) {
+                        ch0 = ((cch[q] >> word) & 1u) != 0u;
+                        ch1 = ((cch[q] >> (4 + word)) & 1u) != 0u;
+                    }
+                }
+            } else {
+                const u32x4 fa = philox10(blk, o0, prun, c3f, k0, k1);
+                const u32x4 fb = philox10(blk, o1, prun, c3f, k0, k1);
+                u0 = pick_word(fa, word);
+                u1 = pick_word(fb, word);
+                uint32_t cb = 0u;
+                if (two_ch) {
+                    const u32x4 ca = philox10(blk, o0, prun, c3c, k0, k1);
+                    const u32x4 cc = philox10(blk, o1, prun, c3c, k0, k1);
+                    cb = (uint32_t)((uint64_t)ca.x < ret_thr) | (uint32_t)((uint64_t)ca.y < ret_thr) << 1 |
+                         (uint32_t)((uint64_t)ca.z < ret_thr) << 2 | (uint32_t)((uint64_t)ca.w < ret_thr) << 3 |
+                         (uint32_t)((uint64_t)cc.x < ret_thr) << 4 | (uint32_t)((uint64_t)cc.y < ret_thr) << 5 |
+                         (uint32_t)((uint64_t)cc.z < ret_thr) << 6 | (uint32_t)((uint64_t)cc.w < ret_thr) << 7;
+                    ch0 = ((cb >> word) & 1u) != 0u;
+                    ch1 = ((cb >> (4 + word)) & 1u) != 0u;
+                }
+                if constexpr (q >= 0) {
+                    cfa[q] = fa;
+                    cfb[q] = fb;
+                    cch[q] = cb;
+                    cache_pairs |= 1u << q;
+                }
             }
             const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
             const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
@@ -286,7 +335,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             nq += __popcll(m1);
             stale |= ((uint64_t)s0 << r) | ((uint64_t)s1 << (r + 1));
             stale_ch |= ((uint64_t)ch0 << r) | ((uint64_t)ch1 << (r + 1));
-        }
+        };
+        pass_a_pair(0, CacheSlot<0>{});
+        if (rows > 2) pass_a_pair(2, CacheSlot<1>{});
+        if (rows > 4) pass_a_pair(4, CacheSlot<2>{});
+        if (rows > 6) pass_a_pair(6, CacheSlot<3>{});
+        for (int r = 2 * kCachePairs; r < rows; r += 2) pass_a_pair(r, CacheSlot<-1>{});
         if (nq > 0) {
             __syncthreads();
             search_queue(nq);
@@ -426,6 +480,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                     base += __popcll(km);
                 }
                 n_slots = base;
+                cache_pairs = 0u;                   // electrons moved to other slots
                 __syncthreads();
             }
         }
