@@ -126,12 +126,6 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     auto claim = [&](uint32_t nn, uint32_t o) {
         atomicMin(&hole_u[4 * nn + 3], (o << 16) | (hole_u[4 * nn + 3] & 0xFFFFu));
     };
-    // Would draw u fire with x = k dt?
-    auto may_fire = [&](uint32_t u, float x) {
-        FireThr t = tA;
-        if (x != xA) t = fire_threshold(x);
-        return t.always || u < t.thr;
-    };
     // Test of one live electron (slot p) with draw u and x = k dt; candidates claim
     // their hole and remember their threshold for a conflict retry.
     auto fire_tail = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
@@ -145,6 +139,28 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
             cand |= 1ull << rr;
         }
+    };
+    // Pass-A test of one live electron (slot p, draw u, x = k dt against its current
+    // w): nothing unless the draw fires; then one 16-byte read of its hole gives
+    // both the removed-hole marker and the claim field.  A removed hole queues the
+    // electron for a new search (its draw kept in ethr; the test against the old w
+    // bounds the one against the new); a live one is claimed.  Returns queued.
+    auto test_fire = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
+                         uint64_t& cand) -> bool {
+        const uint32_t nn = st >> 16;
+        if (nn == kNoHole) return false;
+        FireThr t = tA;
+        if (x != xA) t = fire_threshold(x);
+        if (!(t.always || u < t.thr)) return false;
+        const float4 h = hole[nn];
+        if (h.x == kDeadHoleX) {
+            ethr[p] = u;
+            return true;
+        }
+        atomicMin(&hole_u[4 * nn + 3], (o << 16) | (__float_as_uint(h.w) & 0xFFFFu));
+        ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
+        cand |= 1ull << rr;
+        return false;
     };
     // Search again for the queued electrons rq[0..nq): one electron at a time
     // with the whole wave while the queue is short, lane-parallel otherwise.
@@ -276,7 +292,6 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
             const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
             const float w0 = p0 < n_slots ? ew[p0] : 0.0f, w1 = p1 < n_slots ? ew[p1] : 0.0f;
-            bool s0 = hole_dead(st0 >> 16), s1 = hole_dead(st1 >> 16);
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             // Below x = 2^-32 (in either channel) the threshold is 0: no draw can
             // fire, and a stale electron keeps waiting.  Rows whose lanes all sit
@@ -284,7 +299,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             if (ballot(__builtin_fmaf(bmax, w0, xA) >= 0x1p-32f ||
                        __builtin_fmaf(bmax, w1, xA) >= 0x1p-32f) == 0)
                 return;
-            uint32_t u0, u1;
+            uint32_t u0 = 0u, u1 = 0u;
             bool ch0 = always2, ch1 = always2;
             bool hit = false;
             if constexpr (q >= 0) hit = ((cache_pairs >> q) & 1u) != 0u;
@@ -322,11 +337,8 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             }
             const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
             const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
-            // a queued electron keeps its draw in its (still unused) threshold slot
-            if (!s0) fire_tail(st0, o0, p0, u0, x0, r, cand);
-            else if ((s0 = may_fire(u0, x0))) ethr[p0] = u0;
-            if (!s1) fire_tail(st1, o1, p1, u1, x1, r + 1, cand);
-            else if ((s1 = may_fire(u1, x1))) ethr[p1] = u1;
+            const bool s0 = test_fire(st0, o0, p0, u0, x0, r, cand);
+            const bool s1 = test_fire(st1, o1, p1, u1, x1, r + 1, cand);
             const uint64_t m0 = ballot(s0);
             if (s0) rq[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
             nq += __popcll(m0);
