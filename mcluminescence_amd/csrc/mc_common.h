@@ -161,12 +161,11 @@ __device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, fl
     const int x0 = max(cx - 1, 0), x1 = min(cx + 1, G - 1);
     const int y0 = max(cy - 1, 0), y1 = min(cy + 1, G - 1);
     const int z0 = max(cz - 1, 0), z1 = min(cz + 1, G - 1);
-    const int ny = y1 - y0 + 1;
-    const int nr = (z1 - z0 + 1) * ny;               // <= 9 contiguous slot ranges
     const int j = lane / kPer, o = lane - j * kPer;
+    // (z, y) row j of the unclamped block; rows outside the grid are empty
+    const int jz = j / 3, zz = cz - 1 + jz, yy = cy - 1 + (j - 3 * jz);
     int s = 0, e = 0;
-    if (j < nr) {
-        const int zz = z0 + j / ny, yy = y0 + j % ny;
+    if (j < 9 && (unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
         const int base = (zz * G + yy) * G;
         s = (int)g.cstart[base + x0];
         e = (int)g.cstart[base + x1 + 1];
@@ -216,9 +215,11 @@ __device__ __forceinline__ void block_search_batch(const Grid& g, int nb, float 
         e[b] = 0;
         if (b < nb) {
             const Block k(g, bx[b], by[b], bz[b]);
-            const int ny = k.y1 - k.y0 + 1;
-            if (j < (k.z1 - k.z0 + 1) * ny) {
-                const int base = ((k.z0 + j / ny) * G + k.y0 + j % ny) * G;
+            // (z, y) row j of the unclamped block; rows outside the grid are empty
+            const int cz = cell_of(bz[b], g.inv_cell, G), cy = cell_of(by[b], g.inv_cell, G);
+            const int jz = j / 3, zz = cz - 1 + jz, yy = cy - 1 + (j - 3 * jz);
+            if (j < 9 && (unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
+                const int base = (zz * G + yy) * G;
                 s[b] = (int)g.cstart[base + k.x0];
                 e[b] = (int)g.cstart[base + k.x1 + 1];
             }
