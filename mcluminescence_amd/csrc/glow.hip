@@ -4,7 +4,8 @@
 // kernels' per-step counts.
 //
 // One workgroup per run (grid-stride over runs):
-//   1. max temperature over the first sim's steps with t > 0 (block max);
+//   1. max temperature over the first sim's steps with t > 0 (block max; on the
+//      fixed-dt axis the larger of the first and last step's, T being monotone);
 //   2. the bin count of np.arange(0, max + w, w) edges, the smoothing width and
 //      the curve length, with the too-many-bins / curve-too-long checks;
 //   3. per step: the sum of the run's sims' counts (coalesced, four steps per
@@ -12,10 +13,13 @@
 //      bin np.histogram would choose (edges j * w, last bin closed);
 //   4. a block scan of the histogram and the 'valid' boxcar as differences of
 //      the prefix sums: exact integer window sums, one division each.
-// Bytes per launch: sims x steps x count width + the first sim's time axis +
-// the curves; the histogram never leaves LDS.
+// The histogram is sized to the launch's largest run when the host can tell
+// (fixed-dt axis, or a host time axis), so small curves leave LDS for more
+// workgroups per CU.  Bytes per launch: sims x steps x count width + the first
+// sim's time axis + the curves; the histogram never leaves LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,7 +32,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 
-enum : int32_t { kGlowOk = 0, kGlowTooManyBins = 1, kGlowCurveTooLong = 2 };
+enum : int32_t { kGlowOk = 0, kGlowTooManyBins = 1, kGlowCurveTooLong = 2, kGlowNoRoom = 3 };
 
 struct GlowRun {
     int64_t first;      // first cluster of the run
@@ -45,6 +49,7 @@ struct GlowArgs {
     int32_t count_bytes;    // 4: u32 counts, 1: u8 lum
     int32_t vec4;           // rows allow four-step vector loads
     int32_t k;              // boxcar width in bins
+    int32_t hist_cap;       // histogram bins the launch's LDS holds
     const void* counts;
     int64_t row_stride;
     const double* t_axis;   // [clusters][row_stride] or null (fixed-dt axis)
@@ -150,10 +155,36 @@ __device__ __forceinline__ void step_sum4(const GlowArgs& a, const GlowRun& R, i
     }
 }
 
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int o)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, o, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), o, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Largest T over the steps with t > 0 of the fixed-dt axis t_i = (i + 1) dt,
+// i < n: T is monotone in i, so it is the first or the last step's.
+__host__ __device__ __forceinline__ double fixed_axis_tmax(double T_start, double T_rate, double dt,
+                                                           int64_t n)
+{
+    if (n <= 0) return -INFINITY;
+    const double Ta = T_start + T_rate * ((double)1 * dt);
+    const double Tb = T_start + T_rate * ((double)n * dt);
+    return Tb > Ta ? Tb : Ta;
+}
+
+// Histogram bins of the edges 0, w, 2w, ... below max T + w (np.arange).
+__host__ __device__ __forceinline__ int64_t glow_bins(double tmax, double bw)
+{
+    if (!(tmax > -INFINITY)) return 0;
+    const double len = ceil((tmax + bw) / bw);
+    return len >= 2.0 ? (len < 4.0e18 ? (int64_t)len - 1 : (int64_t)4e18) : 0;
+}
+
 __global__ __launch_bounds__(kThreads) void glow_kernel(GlowArgs a)
 {
-    __shared__ uint64_t hist[MCL_GLOW_MAX_BINS];
-    __shared__ uint64_t part[kThreads];
+    extern __shared__ uint64_t hist[];              // [a.hist_cap]
+    __shared__ uint64_t wsum[kWaves];
     __shared__ double wmax[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int r = blockIdx.x; r < a.n_runs; r += gridDim.x) {
@@ -162,7 +193,9 @@ __global__ __launch_bounds__(kThreads) void glow_kernel(GlowArgs a)
                                    : (R.n_steps < a.row_stride ? (int64_t)R.n_steps : a.row_stride);
         // 1. max T over steps with t > 0 (-inf when there are none)
         double tmax = -INFINITY;
-        if (R.sims > 0) {
+        if (R.sims > 0 && !a.t_axis) {
+            tmax = fixed_axis_tmax(R.T_start, R.T_rate, R.dt, n);
+        } else if (R.sims > 0) {
             for (int64_t i = tid; i < n; i += kThreads) {
                 const double t = t_of(a, R, i);
                 if (t > 0.0) {
@@ -170,22 +203,19 @@ __global__ __launch_bounds__(kThreads) void glow_kernel(GlowArgs a)
                     tmax = T > tmax ? T : tmax;
                 }
             }
+            tmax = wave_max_d(tmax);
+            if (lane == 0) wmax[wave] = tmax;
+            __syncthreads();
+            tmax = wmax[0];
+            for (int w = 1; w < kWaves; ++w) tmax = wmax[w] > tmax ? wmax[w] : tmax;
         }
-        tmax = wave_max_d(tmax);
-        if (lane == 0) wmax[wave] = tmax;
-        __syncthreads();
-        tmax = wmax[0];
-        for (int w = 1; w < kWaves; ++w) tmax = wmax[w] > tmax ? wmax[w] : tmax;
         // 2. edges 0, w, 2w, ... below max + w: ceil((max + w) / w) of them
-        int64_t n_bins = 0;
-        if (tmax > -INFINITY) {
-            const double len = std::ceil((tmax + a.bw) / a.bw);
-            n_bins = len >= 2.0 ? (len < 4.0e18 ? (int64_t)len - 1 : (int64_t)4e18) : 0;
-        }
+        const int64_t n_bins = glow_bins(tmax, a.bw);
         int64_t L = 0;
         if (n_bins > 0) L = n_bins >= a.k ? n_bins - a.k + 1 : (int64_t)a.k - n_bins + 1;
         int32_t st = kGlowOk;
         if (n_bins > MCL_GLOW_MAX_BINS) st = kGlowTooManyBins;
+        else if (n_bins > a.hist_cap) st = kGlowNoRoom;
         else if (L > a.curve_stride) st = kGlowCurveTooLong;
         if (tid == 0) {
             a.status[r] = st;
@@ -234,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void glow_kernel(GlowArgs a)
         }
         __syncthreads();
         // 4. inclusive prefix sums of the histogram: per-thread chunk, then the
-        //    chunk totals scanned by thread 0 (256 entries)
+        //    chunk totals scanned within each wave and across the four waves
         const int chunk = (nb + kThreads - 1) / kThreads;
         const int j0 = tid * chunk, j1 = j0 + chunk < nb ? j0 + chunk : nb;
         uint64_t run = 0;
@@ -242,18 +272,16 @@ __global__ __launch_bounds__(kThreads) void glow_kernel(GlowArgs a)
             run += hist[j];
             hist[j] = run;
         }
-        part[tid] = run;
-        __syncthreads();
-        if (tid == 0) {
-            uint64_t acc = 0;
-            for (int q = 0; q < kThreads; ++q) {
-                const uint64_t v = part[q];
-                part[q] = acc;
-                acc += v;
-            }
+        uint64_t incl = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t u = shfl_up_u64(incl, o);
+            if (lane >= o) incl += u;
         }
+        if (lane == 63) wsum[wave] = incl;
         __syncthreads();
-        const uint64_t off = part[tid];
+        uint64_t off = incl - run;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
         for (int j = j0; j < j1; ++j) hist[j] += off;
         __syncthreads();
         // boxcar: window sums P[m + k - 1] - P[m - 1]; k > n_bins: every output is the total
@@ -315,6 +343,32 @@ extern "C" int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_r
     }
     if (total <= 0) return mcl::fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
     const bool dev = (flags & MCL_F_DEVICE_PTRS) != 0;
+    // histogram bins of the largest run, when the time axis is known here (the
+    // kernel's own arithmetic, so the same count); a device time axis gets the most
+    int64_t cap = 1;
+    if (dev && t_axis) {
+        cap = MCL_GLOW_MAX_BINS;
+    } else {
+        for (int r = 0; r < n_runs; ++r) {
+            const GlowRun& g = gr[(size_t)r];
+            if (g.sims <= 0) continue;
+            double tmax = -INFINITY;
+            if (!t_axis) {
+                tmax = fixed_axis_tmax(g.T_start, g.T_rate, g.dt,
+                                       g.n_steps < row_stride ? (int64_t)g.n_steps : row_stride);
+            } else {
+                const double* row = t_axis + g.first * row_stride;
+                for (int64_t i = 0; i < row_stride; ++i) {
+                    if (!(row[i] > 0.0)) continue;
+                    const double T = g.T_start + g.T_rate * row[i];
+                    tmax = T > tmax ? T : tmax;
+                }
+            }
+            const int64_t nbins = glow_bins(tmax, bin_width);
+            cap = std::max(cap, std::min(nbins, (int64_t)MCL_GLOW_MAX_BINS));
+        }
+    }
+    const size_t lds_bytes = sizeof(uint64_t) * (size_t)cap;
     if (hipSetDevice(c->device) != hipSuccess)
         return mcl::fail(c, MCL_EDEVICE, "hipSetDevice failed");
     const size_t cnt_bytes = (size_t)total * (size_t)row_stride * (size_t)count_bytes;
@@ -358,6 +412,7 @@ extern "C" int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_r
     a.count_bytes = count_bytes;
     a.vec4 = (row_stride % 4 == 0) && ((uintptr_t)d_counts % 16 == 0);
     a.k = k;
+    a.hist_cap = (int32_t)cap;
     a.counts = d_counts;
     a.row_stride = row_stride;
     a.t_axis = d_t;
@@ -370,9 +425,12 @@ extern "C" int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_r
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     if (e == hipSuccess) e = hipEventRecord(ev0, c->stream);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)glow_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e == hipSuccess) {
         const int grid = n_runs < 65536 ? n_runs : 65536;
-        hipLaunchKernelGGL(glow_kernel, dim3(grid), dim3(kThreads), 0, c->stream, a);
+        hipLaunchKernelGGL(glow_kernel, dim3(grid), dim3(kThreads), lds_bytes, c->stream, a);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(ev1, c->stream);
@@ -404,6 +462,9 @@ extern "C" int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_r
         if (hst[(size_t)r] == kGlowTooManyBins)
             return mcl::fail(c, MCL_ERANGE, "run %d: the histogram has more than %d bins of "
                              "width %g", r, MCL_GLOW_MAX_BINS, bin_width);
+        if (hst[(size_t)r] == kGlowNoRoom)
+            return mcl::fail(c, MCL_EDEVICE, "run %d: histogram larger than the launch's LDS "
+                             "(%lld bins)", r, (long long)cap);
         if (hst[(size_t)r] == kGlowCurveTooLong)
             return mcl::fail(c, MCL_EOVERFLOW, "run %d: the curve has %d points; curve_stride "
                              "is %lld", r, hlen[(size_t)r], (long long)curve_stride);
