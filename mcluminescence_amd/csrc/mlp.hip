@@ -25,7 +25,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
-constexpr int kBK = 32;    // K per LDS stage
+constexpr int kBK = 64;    // K per LDS stage
 constexpr int kPad = 8;    // LDS row padding [bf16]: 80-byte rows, conflict-free 16-byte reads
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }

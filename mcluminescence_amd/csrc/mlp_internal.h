@@ -30,7 +30,7 @@ struct GemmArgs {
     const __bf16* A;          // [M][lda], this split's K range starts at z * K
     const __bf16* B;          // [N][ldb]
     int64_t lda, ldb;
-    int M, N, K;              // K = contraction length of one split (multiple of 32)
+    int M, N, K;              // K = contraction length of one split (multiple of 64)
     const float* bias;        // [N]
     __bf16* out_a;            // [M][N]
     __bf16* out_z;            // [M][N]
