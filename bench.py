@@ -67,6 +67,109 @@ def cpu_baseline(seconds: float) -> dict:
             "curves_per_s": done / el}
 
 
+def cpu_event_lab(seconds: float) -> dict:
+    """The numpy restatements of the event loop (simulate.py:460-516) and of TL_lab
+    (tl_trap_lab.py:65-107) on the workloads of event_lab_extras, one core, each for
+    about `seconds`."""
+    import numpy as np
+    from oracle import event_mc as E
+    from oracle import lab_mc
+    run = event_workload(sims=1)[0]
+    exp, phys = E.run_params_to_cfg(run)
+    exp["duration"] = EVENT_DURATION
+    exp["max_dt"] = run.dt
+    done, steps = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        x, _, _ = E.simulate_run(exp, phys, sims=1, steps=4096, seed=0xE7E47, first_sim=done)
+        steps += int((x[:, 0] > 0).sum())
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    event = {"value": steps / el, "unit": "event steps/s", "cores": 1, "kind": "port",
+             "sample": f"{done} clusters of the event workload in {el:.1f} s, numpy "
+                       "restatement of simulate.py:460-516"}
+    lexp, lphys, rows = lab_workload()
+    rng = np.random.default_rng(11)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        lab_mc.tl_lab_fractions(lexp, lphys, rows, rng)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    lab = {"value": done * len(rows) / el, "unit": "lab experiments/s", "cores": 1,
+           "kind": "port", "sample": f"{done} passes over the {len(rows)} TL rows in {el:.1f} s, "
+                                     "numpy restatement of TL_lab (tl_trap_lab.py:65-107)"}
+    return {"event": event, "lab": lab}
+
+
+EVENT_DURATION = 160.0      # s: 0 -> 800 degC at 5 K/s
+
+
+def event_workload(sims: int = 4096):
+    """The reference's TL12 physics (tunnelling only) at 400 electrons, 5 K/s, step
+    cap max_dt / T_rate = 0.2 s: simulate.py's own loop, one event per step."""
+    from mcluminescence_amd import workloads as W
+    runs = W.tunnelling_tl12(sims=sims, n=400, T_rate=5.0)
+    for p in runs:
+        p.dt = 1.0                               # max_dt (the event kernel caps dt at it / T_rate)
+    return runs
+
+
+def lab_workload():
+    """TL_lab on the CLBR_IRSL50 table (11 rows) with the shipped lab configs."""
+    import json as _json
+    from mcluminescence_amd import config, engine
+    from oracle import event_mc as E
+    t = _json.loads((ROOT / "tests" / "golden" / "reference_outputs.json").read_text())[
+        "lab_data"]["CLBR_IRSL50_0.25KperGy"]
+    cfg = engine.initialize_runs(config.compose(["exp_type_fp=TLlab", "physics_fp=lab_TL"]))[0]
+    p = engine.run_params(cfg, sims=1, n_steps=1)
+    exp, phys = E.run_params_to_cfg(p)
+    phys["D"] = float(cfg.physics_fp.D)
+    rows = [dict(Duration=t["Duration"][k], T_start=t["T_start"][k], T_end=t["T_end"][k],
+                 e_ratio=t["e_ratio"][k]) for k in range(len(t["Duration"]))]
+    return exp, phys, rows
+
+
+def event_lab_extras(ctx) -> dict:
+    """Event mode (4096 clusters of the event workload) and the lab kernel (TL_lab on
+    the CLBR table, 4096 simulations per row): kernel time from HIP events, steps
+    simulated per second."""
+    import numpy as np
+    from mcluminescence_amd import config, engine, lab
+    runs = event_workload()
+    out = {}
+    ms, st = [], None
+    for _ in range(4):
+        _, _, _, steps, st = ctx.run_event(runs, [EVENT_DURATION], max_steps=4096)
+        ms.append(st.kernel_ms)
+    m = float(np.median(ms[1:]))
+    out["event"] = {"workload": "TL12 physics, 4096 clusters x 400 electrons, 0 -> 800 degC at 5 K/s",
+                    "kernel_ms": m, "event_steps": int(st.cluster_steps),
+                    "event_steps_per_s": st.cluster_steps / (m * 1e-3),
+                    "recombinations": int(st.recombinations)}
+    cfg = engine.initialize_runs(config.compose(["exp_type_fp=TLlab", "physics_fp=lab_TL"]))[0]
+    import json as _json
+    table = _json.loads((ROOT / "tests" / "golden" / "reference_outputs.json").read_text())[
+        "lab_data"]["CLBR_IRSL50_0.25KperGy"]
+    lruns, obs = lab.tl_lab_runs(cfg, table, sims=4096)
+    ms = []
+    for _ in range(4):
+        _, _, st = ctx.run_lab(lruns, obs)
+        ms.append(st.kernel_ms)
+    m = float(np.median(ms[1:]))
+    n_exp = sum(int(r.sims) for r in lruns)
+    out["lab"] = {"workload": "TL_lab, CLBR_IRSL50 table (11 rows) x 4096 simulations, lab_TL physics",
+                  "kernel_ms": m, "experiments": n_exp, "experiments_per_s": n_exp / (m * 1e-3),
+                  "event_steps": int(st.cluster_steps),
+                  "event_steps_per_s": st.cluster_steps / (m * 1e-3)}
+    return out
+
+
 PEAK_BF16_TFLOPS = 2500.0       # dense BF16 matrix-core peak
 
 
@@ -303,9 +406,12 @@ def main() -> int:
     if not args.no_extras:
         ctx.set_stream(None)
         result["tl"] = tl_extras(ctx, stream)
+        result.update(event_lab_extras(ctx))
         result["surrogate"] = surrogate_extras(ctx)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        if not args.no_extras:
+            result["cpu_baseline"].update(cpu_event_lab(args.cpu_seconds / 3))
         result["cpu_baseline"]["host_cpu"] = _cpu_model()
         result["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
     print(json.dumps(result), flush=True)
