@@ -14,7 +14,8 @@ roofline   = the MC kernel against the VALU issue rate of the chip: algorithmic
              lane-slots per launch (per live electron-step 1/4 Philox4x32-10
              block + fire test = 28 slots; per electron-hole distance 9 slots)
              divided by the kernel's mean launch time from HIP events recorded
-             on the launch stream.
+             on the launch stream.  roofline_hbm = the same kernel against HBM
+             (the counts it must write), which is not what binds it.
 cpu_baseline = the numpy event-mode restatement of the reference loop
              (simulate.py:460-516, one recombination per step, dense distance
              matrix) on a bounded sample of the same clusters, one core,
@@ -401,6 +402,12 @@ def main() -> int:
                      "kernel_ms": mean_ms,
                      "hbm_gbps_algorithmic": out_bytes / (mean_ms * 1e-3) / 1e9,
                      "hbm_peak_gbps": PEAK_HBM_GBPS},
+        # the same kernel against the HBM roof: algorithmic bytes (the counts and
+        # live-step words it must write) over the same mean launch time
+        "roofline_hbm": {"bound": "hbm", "achieved": out_bytes / (mean_ms * 1e-3) / 1e9,
+                         "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                         "frac": out_bytes / (mean_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS,
+                         "traffic": TRAFFIC_BYTES_DECAY if args.sims == 4096 else None},
     }
     plan.close()
     if not args.no_extras:
