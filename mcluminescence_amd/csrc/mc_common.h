@@ -289,6 +289,41 @@ __device__ __forceinline__ uint64_t nn_resolve(const Grid& g, bool act, float x,
     return key;
 }
 
+// Inclusive prefix sum of a float64 value over the 64 lanes (all active), in a
+// fixed order the CPU replay repeats: within each 16-lane row, steps of 1, 2, 4
+// and 8 lanes (v[l] += v[l - o] for l % 16 >= o, by DPP row shifts); then rows
+// 1, 2, 3 add t0, t0 + t1 and (t0 + t1) + t2, the totals t_r of the rows before.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_row_shift_d(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, 0xF, 0xF, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, 0xF,
+                                                              0xF, true);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double lane_value_d(double v, int l)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t r = ((uint64_t)lane_value((uint32_t)(b >> 32), l) << 32) |
+                       lane_value((uint32_t)b, l);
+    return __longlong_as_double((long long)r);
+}
+
+__device__ __forceinline__ double row_scan_d(double v, int lane)
+{
+    v = v + dpp_row_shift_d<0x111>(v);    // row_shr:1 (0 shifted in)
+    v = v + dpp_row_shift_d<0x112>(v);    // row_shr:2
+    v = v + dpp_row_shift_d<0x114>(v);    // row_shr:4
+    v = v + dpp_row_shift_d<0x118>(v);    // row_shr:8
+    const double t0 = lane_value_d(v, 15), t1 = lane_value_d(v, 31), t2 = lane_value_d(v, 47);
+    const double o2 = t0 + t1, o3 = o2 + t2;
+    const int row = lane >> 4;
+    if (row > 0) v = v + (row == 1 ? t0 : row == 2 ? o2 : o3);
+    return v;
+}
+
 // w = exp(-alpha r) of a squared distance (every path computes it this one way).
 __device__ __forceinline__ float w_of(uint32_t d2bits, float alpha_f)
 {

@@ -11,7 +11,8 @@
 //   (index order) k_e = fma(B_ch, w_e, A) in float64, the channel drawn as in the
 //   fixed-dt step (Philox (i >> 2, e, run, sim|CHAN), word i & 3).
 //   K = sum of k_e: each row of 64 consecutive live electrons is summed by an
-//   inclusive scan over offsets 1, 2, 4, ..., 32, and the rows are added in order.
+//   inclusive scan (row_scan_d: steps of 1, 2, 4, 8 within 16-lane rows, then the
+//   totals of the rows before), and the rows are added in order.
 //   One Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives three words:
 //   E_t = -ln((u0 + 1/2) 2^-32), U_w = u1 2^-32, E_f = -ln((u2 + 1/2) 2^-32).
 //   dt_fill = E_f (1 / lambda), lambda = 1e-20 when D = 0 or n_e = N_e
@@ -34,16 +35,6 @@
 
 namespace mcl {
 
-// Inclusive prefix sum over the 64 lanes (offsets 1, 2, ..., 32).
-__device__ __forceinline__ double row_scan(double v, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double u = __shfl_up(v, o, kWave);
-        if (lane >= o) v = v + u;
-    }
-    return v;
-}
 
 __device__ __forceinline__ double exp1_of(uint32_t u)
 {
@@ -124,7 +115,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         const int rows = (n_live + kWave - 1) / kWave;
         double K = 0.0;
         for (int r = 0; r < rows; ++r) {
-            const double v = row_scan(rate_of(r * kWave + lane), lane);
+            const double v = row_scan_d(rate_of(r * kWave + lane), lane);
             const double rt = __shfl(v, kWave - 1, kWave);
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
@@ -154,7 +145,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             for (int r = 0; r < rows; ++r) {
                 const double c2 = carry + rtot[r];
                 if (c2 > target) {
-                    const double v = row_scan(rate_of(r * kWave + lane), lane);
+                    const double v = row_scan_d(rate_of(r * kWave + lane), lane);
                     const uint64_t m = ballot(carry + v > target);
                     pw = r * kWave + (int)__builtin_ctzll(m);
                     break;

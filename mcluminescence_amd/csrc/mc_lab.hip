@@ -47,15 +47,6 @@ namespace {
 constexpr uint32_t kFillCtr = 0xFFFFFFFEu;
 constexpr uint32_t kSharedChanCtr = 0xFFFFFFFDu;
 
-__device__ __forceinline__ double lab_scan(double v, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double u = __shfl_up(v, o, kWave);
-        if (lane >= o) v = v + u;
-    }
-    return v;
-}
 
 __device__ __forceinline__ double lab_exp1(uint32_t u)
 {
@@ -246,7 +237,7 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
         const int rows = (n_e + kWave - 1) / kWave;
         double K = 0.0;
         for (int r = 0; r < rows; ++r) {
-            const double v = lab_scan(rate_of(r * kWave + lane), lane);
+            const double v = row_scan_d(rate_of(r * kWave + lane), lane);
             const double rt = __shfl(v, kWave - 1, kWave);
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
@@ -281,7 +272,7 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                 for (int r = 0; r < rows; ++r) {
                     const double c2 = carry + rtot[r];
                     if (c2 > target) {
-                        const double v = lab_scan(rate_of(r * kWave + lane), lane);
+                        const double v = row_scan_d(rate_of(r * kWave + lane), lane);
                         const uint64_t m = ballot(carry + v > target);
                         pw = r * kWave + (int)__builtin_ctzll(m);
                         break;

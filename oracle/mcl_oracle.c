@@ -650,8 +650,9 @@ uint64_t orc_fire_threshold(float x) { return thr_f32(x); }
  * reference's loop simulate.py:46-92 with one event per step:
  *   T = (T_start + T_rate t) + 273.15; A, B_c in float64 (own exp);
  *   k_e = fma(B_ch, w_e, A) per live electron in index order;
- *   K = rows of 64 electrons, each an inclusive scan over offsets 1..32, summed
- *   in order; one Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives
+ *   K = rows of 64 electrons, each an inclusive scan (steps of 1, 2, 4, 8 within
+ *   16-lane rows, then rows 1..3 add t0, t0 + t1, (t0 + t1) + t2 of the rows
+ *   before), summed in order; one Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives
  *   E_t = -ln((u0 + 1/2) 2^-32), U_w = u1 2^-32, E_f = -ln((u2 + 1/2) 2^-32);
  *   dt_fill = (1 / lambda) E_f, dt_rec = E_t / K (dt_fill without electrons, +inf
  *   when K = 0), dt = min(dt_fill, dt_rec, dt_cap); a recombination takes the
@@ -659,10 +660,14 @@ uint64_t orc_fire_threshold(float x) { return thr_f32(x); }
 static void scan64(double* v)
 {
     double old[64];
-    for (int o = 1; o < 64; o <<= 1) {
+    for (int o = 1; o < 16; o <<= 1) {
         memcpy(old, v, sizeof old);
-        for (int l = o; l < 64; ++l) v[l] = old[l] + old[l - o];
+        for (int l = 0; l < 64; ++l)
+            if ((l & 15) >= o) v[l] = old[l] + old[l - o];
     }
+    const double t0 = v[15], t1 = v[31], t2 = v[47];
+    const double o2 = t0 + t1, o3 = o2 + t2;
+    for (int l = 16; l < 64; ++l) v[l] = v[l] + (l < 32 ? t0 : l < 48 ? o2 : o3);
 }
 
 typedef struct {
