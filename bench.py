@@ -208,6 +208,92 @@ def tl_extras(ctx, stream) -> dict:
                               "1024 parameter sets per call, host round trip included"}
 
 
+def sweep_extras(ctx) -> dict:
+    """The uneven batched sweep: 10,000 parameter sets (half TL at 0.5-5 K/s over
+    300-750 K, half isothermal at 400-600 K; 64-2048 electrons; 1024 steps) as one
+    persistent, graph-captured launch with the tuning table's knobs, 256-bin curves
+    out; and the same sets as one workgroup per cluster for comparison.  Device time
+    of a launch from HIP events around it."""
+    import numpy as np
+    import torch
+    from mcluminescence_amd import autotune
+    from mcluminescence_amd import workloads as W
+    from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT
+    runs = W.uneven_sweep()
+    knobs = autotune.apply(ctx, runs)
+    out = {"workload": "uneven sweep: 10,000 sets (5,000 TL, 5,000 isothermal), 64-2048 "
+                       "electrons, 1024 steps, 256-bin curves",
+           "sets": len(runs), "knobs": knobs,
+           "tuned": autotune.signature(runs) in autotune.load_table()}
+    ref = None
+    for label, flags in (("persistent_graph", F_PERSISTENT | F_GRAPH), ("one_wg_per_cluster", 0)):
+        plan = ctx.plan(runs, n_bins=256, flags=flags)
+        n, row = plan.info.total_clusters, plan.info.row_len
+        counts = torch.zeros((n, row), dtype=torch.int32, device="cuda")
+        live = torch.zeros(n, dtype=torch.int64, device="cuda")
+        plan.launch(counts.data_ptr(), live.data_ptr())
+        plan.sync()
+        ms = []
+        for _ in range(3):
+            plan.launch(counts.data_ptr(), live.data_ptr())
+            plan.sync()
+            ms.append(plan.last_ms())
+        w = plan.work()
+        got = counts.cpu().numpy()
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(ref, got), "persistent and plain launches disagree"
+        plan.close()
+        m = float(np.median(ms))
+        slots = w.live_electron_steps * SLOTS_PER_LIVE_ESTEP + w.distance_evals * SLOTS_PER_DISTANCE
+        out[label] = {"ms": m, "curves_per_s": n / (m * 1e-3),
+                      "live_electron_steps_per_s": w.live_electron_steps / (m * 1e-3),
+                      "valu_frac": slots / (m * 1e-3) / PEAK_VALU_LANE_SLOTS}
+        out["live_electron_steps"] = int(w.live_electron_steps)
+        out["distance_evals"] = int(w.distance_evals)
+        out["recombinations"] = int(w.recombinations)
+    autotune.apply(ctx, [], {})                  # back to the default knobs
+    return out
+
+
+def cpu_harness(seconds: float) -> dict:
+    """The C replay harness (the fixed-dt algorithm the kernels run, OpenMP over
+    clusters) on bounded samples: the decay clusters on one core and on every host
+    core (at most 16), and the first sets of the uneven sweep on every core."""
+    from oracle import orc
+    from mcluminescence_amd import workloads as W
+    cores = min(16, os.cpu_count() or 1)
+    out = {}
+    for label, th in (("decay_1core", 1), ("decay_allcores", cores)):
+        sims, el, live = 16, 0.0, 0
+        while True:
+            t0 = time.perf_counter()
+            _, lv = orc.run_fixed_dt(W.iso_decay_4096x400(sims=sims), threads=th)
+            el = time.perf_counter() - t0
+            live = int(lv.sum())
+            if el >= seconds / 4 or sims >= 4096:
+                break
+            sims = min(4096, sims * 4)
+        out[label] = {"value": live / el, "unit": "electron-steps/s", "cores": th,
+                      "sample": f"{sims} of the 4096 decay clusters in {el:.2f} s"}
+    runs = W.uneven_sweep()
+    k = 64
+    while True:
+        t0 = time.perf_counter()
+        _, lv = orc.run_fixed_dt(runs[:k], n_bins=256, threads=cores)
+        el = time.perf_counter() - t0
+        if el >= seconds / 4 or k >= len(runs):
+            break
+        k = min(len(runs), k * 4)
+    out["sweep_allcores"] = {"value": k / el, "unit": "curves/s", "cores": cores,
+                             "electron_steps_per_s": int(lv.sum()) / el,
+                             "sample": f"the first {k} of the 10,000 sweep sets in {el:.2f} s"}
+    out["kind"] = "port"
+    out["what"] = "oracle/mcl_oracle.c fixed-dt replay harness (gcc -O2, OpenMP)"
+    return out
+
+
 def glow_extras(ctx, runs, counts, batch: int = 1024) -> dict:
     """The glow-curve kernel (1-degree bins, 50-degree boxcar) on the 16 x 500 glow's
     per-step counts, alone and as a batch of `batch` copies of that run (HBM-bound:
@@ -304,6 +390,9 @@ def main() -> int:
                     help="skip the glow-curve and surrogate figures")
     ap.add_argument("--surrogate-only", action="store_true",
                     help="only time the surrogate network (training step, inference batch)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only time the uneven 10,000-set sweep (and its CPU harness leg "
+                         "unless --no-cpu)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB three times (the "
                          "reference point for hardware byte counters) and exit")
@@ -313,6 +402,15 @@ def main() -> int:
         ctx = Context(0)
         print(json.dumps(surrogate_extras(ctx, fits=False)), flush=True)
         ctx.close()
+        return 0
+    if args.sweep_only:
+        from mcluminescence_amd._abi import Context
+        ctx = Context(0)
+        res = {"sweep": sweep_extras(ctx)}
+        ctx.close()
+        if not args.no_cpu:
+            res["cpu_harness"] = cpu_harness(args.cpu_seconds)
+        print(json.dumps(res), flush=True)
         return 0
     if args.copy_gib > 0:
         import torch
@@ -332,6 +430,7 @@ def main() -> int:
         return 2
 
     import torch
+    from mcluminescence_amd import autotune
     from mcluminescence_amd import workloads as W
     from mcluminescence_amd._abi import Context
 
@@ -343,6 +442,7 @@ def main() -> int:
     ctx = Context(0)
     ctx.set_stream(stream.cuda_stream)
     runs = W.iso_decay_4096x400(sims=args.sims)
+    knobs = autotune.apply(ctx, runs)         # the tuning table's knobs (defaults if untuned)
     plan = ctx.plan(runs)                     # per-step counts: n_bins = 0, rows of n_steps
     n, nb = plan.info.total_clusters, plan.info.row_len
     counts = torch.zeros((n, nb), dtype=torch.int32, device=dev)
@@ -389,7 +489,7 @@ def main() -> int:
         "config": {"workload": "iso_decay_500K_4096x400", "clusters": n, "electrons": 400,
                    "holes": 691, "T_K": 500.0, "dt_s": 0.5, "n_steps": 4000, "E_eV": 1.4,
                    "s_per_s": 1e13, "alpha_per_m": 4e9, "rho_per_m3": 1e23,
-                   "kernel": "fixed_dt_kernel (1 wave64 per cluster)"},
+                   "kernel": "fixed_dt_kernel (1 wave64 per cluster)", "knobs": knobs},
         "work_per_step": {"live_electron_steps": w.live_electron_steps,
                           "distance_evals": w.distance_evals,
                           "recombinations": w.recombinations,
@@ -413,12 +513,14 @@ def main() -> int:
     if not args.no_extras:
         ctx.set_stream(None)
         result["tl"] = tl_extras(ctx, stream)
+        result["sweep"] = sweep_extras(ctx)
         result.update(event_lab_extras(ctx))
         result["surrogate"] = surrogate_extras(ctx)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         if not args.no_extras:
             result["cpu_baseline"].update(cpu_event_lab(args.cpu_seconds / 3))
+            result["cpu_baseline"]["harness"] = cpu_harness(args.cpu_seconds / 3)
         result["cpu_baseline"]["host_cpu"] = _cpu_model()
         result["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
     print(json.dumps(result), flush=True)

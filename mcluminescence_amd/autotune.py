@@ -27,6 +27,7 @@ import math
 from pathlib import Path
 from typing import Dict, Iterable, List, Optional, Sequence
 
+from . import workloads as W
 from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geometry
 
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
@@ -127,3 +128,43 @@ def autotune(ctx: Context, runs: Sequence[RunParams], n_bins: int = 0,
     for k, v in best.items():
         ctx.tune_set(k, v)
     return entry
+
+
+# named workloads for the command line: (run list, n_bins)
+WORKLOADS = {
+    "uneven_sweep": (W.uneven_sweep, 256),
+    "iso_decay_4096x400": (W.iso_decay_4096x400, 0),
+    "tl_glow_16x500": (W.tl_glow_16x500, 4000),
+}
+
+
+def main(argv: Optional[Iterable[str]] = None) -> int:
+    """python -m mcluminescence_amd.autotune --workload uneven_sweep [--table PATH]"""
+    import argparse
+    ap = argparse.ArgumentParser(description="time the launch knobs on a named workload and "
+                                             "record the fastest in the tuning table")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="uneven_sweep")
+    ap.add_argument("--table", type=Path, default=TABLE)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--plain", action="store_true",
+                    help="tune the one-workgroup-per-cluster launch instead of the persistent, "
+                         "graph-captured one")
+    args = ap.parse_args(None if argv is None else list(argv))
+    make, n_bins = WORKLOADS[args.workload]
+    runs = make()
+    flags = 0 if args.plain else F_PERSISTENT | F_GRAPH
+    ctx = Context(0)
+    try:
+        base = time_config(ctx, runs, DEFAULTS, n_bins, flags, args.reps)
+        print(f"defaults {DEFAULTS}: {base:.3f} ms", flush=True)
+        entry = autotune(ctx, runs, n_bins=n_bins, flags=flags, reps=args.reps,
+                         table_path=args.table, log=lambda m: print(m, flush=True))
+        print(json.dumps({"workload": args.workload, "signature": signature(runs),
+                          "flags": flags, "defaults_ms": base, **entry}), flush=True)
+    finally:
+        ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -28,7 +28,7 @@ from typing import Callable, Optional, Sequence, Tuple
 import numpy as np
 
 from . import workloads as W
-from ._abi import DEFAULT_SEED, AdamW, Context, MclError, MlpDesc, RunParams
+from ._abi import DEFAULT_SEED, F_PERSISTENT, AdamW, Context, MclError, MlpDesc, RunParams
 
 N_IN = 6
 N_BINS = 256
@@ -106,13 +106,15 @@ def tl_run(theta: Sequence[float], n: int, sims: int) -> RunParams:
 def mc_curves(ctx: Context, theta: np.ndarray, n: int = 400, sims: int = 8,
               seed: int = DEFAULT_SEED, chunk: int = 8192) -> np.ndarray:
     """Monte Carlo glow curves [N, 256]: recombinations per bin summed over the
-    sims, divided by sims * N_e."""
+    sims, divided by sims * N_e.  The parameter sets are uneven in work, so each chunk
+    runs as one persistent launch (clusters taken largest expected work first; the
+    counts equal those of a one-workgroup-per-cluster launch)."""
     theta = np.atleast_2d(theta)
     out = np.empty((theta.shape[0], N_BINS), np.float64)
     for c0 in range(0, theta.shape[0], chunk):
         part = theta[c0:c0 + chunk]
         runs = [tl_run(t, n, sims) for t in part]
-        counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS)
+        counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=F_PERSISTENT)
         out[c0:c0 + len(part)] = counts.reshape(len(part), sims, N_BINS).sum(axis=1) / (sims * n)
     return out
 
