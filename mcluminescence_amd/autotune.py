@@ -12,6 +12,8 @@ The kernels' results do not depend on the tuning knobs (mcl_tune_set):
   persist_frac     persistent workgroups per size class, as a fraction of what
                    fits on the device at once (load balance against contention
                    between concurrently running class kernels)
+  waves_per_cluster  wavefronts per cluster (1, 2, 4; 0 lets each launch or size
+                   class pick the count that keeps the most waves resident)
 
 so the tuner times every candidate on the workload itself (a plan launch on
 device buffers, HIP-event time of the launch), keeps the fastest, and records the
@@ -32,9 +34,10 @@ from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geome
 
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
 DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 32.0,
-            "lookahead_max": 8.0}
+            "lookahead_max": 8.0, "waves_per_cluster": 0.0}
 SPACE = {"holes_per_cell": [2.0, 3.0, 4.0], "persist_frac": [1.0, 2.0],
-         "wave_search_max": [8.0, 32.0, 64.0], "lookahead_max": [0.0, 8.0, 32.0]}
+         "wave_search_max": [8.0, 32.0, 64.0], "lookahead_max": [0.0, 8.0, 32.0],
+         "waves_per_cluster": [0.0, 1.0]}
 
 
 def signature(runs: Sequence[RunParams]) -> str:

@@ -349,22 +349,31 @@ __device__ __forceinline__ uint64_t rows_any(uint64_t bits, int rows)
 
 // Holes of one cluster counting-sorted by grid cell (Box.seed, engine.py:124-129)
 // and every electron's initial nearest hole (Box._rebuild, engine.py:113-119):
-// fills hole, cstart, est and ew.  Shared by the fixed-dt and event kernels.
-__device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGen& egen,
-                                              float4* hole, uint32_t* cstart, uint32_t* est,
-                                              float* ew, uint32_t prun, uint32_t psim,
-                                              uint32_t k0, uint32_t k1, int lane,
-                                              uint64_t& dist_evals)
+// fills hole, cstart, est and ew.  NW wavefronts per workgroup (tid = thread
+// index): the hole counts and the electrons' searches are split over the waves;
+// the cell prefix, the scatter (slots within a cell in hole-index order) and the
+// shift are wave 0's, between the workgroup's barriers.  Shared by the fixed-dt
+// and event kernels.
+template <int NW>
+__device__ __forceinline__ Grid setup_cluster_t(const RunDesc& R, const ElectronGen& egen,
+                                                float4* hole, uint32_t* cstart, uint32_t* est,
+                                                float* ew, uint32_t prun, uint32_t psim,
+                                                uint32_t k0, uint32_t k1, int tid,
+                                                uint64_t& dist_evals)
 {
+    constexpr int NT = NW * kWave;
+    const int lane = NW == 1 ? tid : (tid & (kWave - 1));
+    const bool w0 = NW == 1 || tid < kWave;
+    const int wv = NW == 1 ? 0 : uniform(tid >> 6);
     const int n_e0 = uniform(R.n_e0);
     const int n_h = uniform(R.n_h);
     const int G = uniform(R.grid);
     const int n_cells = G * G * G;
     const float cell = R.cell_nm, inv_cell = R.inv_cell;
     // ---- holes (Box.seed, engine.py:124-129), counting-sorted by grid cell ------
-    for (int i = lane; i <= n_cells; i += kWave) cstart[i] = 0u;
+    for (int i = tid; i <= n_cells; i += NT) cstart[i] = 0u;
     __syncthreads();
-    for (int h = lane; h < n_h; h += kWave) {
+    for (int h = tid; h < n_h; h += NT) {
         const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
         const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
                     z = unit_coord(o.z, R.side_b_nm);
@@ -373,7 +382,7 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
         atomicAdd(&cstart[cc + 1], 1u);
     }
     __syncthreads();
-    {
+    if (w0) {
         uint32_t carry = 0;
         for (int base = 0; base <= n_cells; base += kWave) {
             const int i = base + lane;
@@ -389,22 +398,24 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
         }
     }
     __syncthreads();
-    for (int h = lane; h < n_h; h += kWave) {
-        const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
-        const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
-                    z = unit_coord(o.z, R.side_b_nm);
-        const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
-                       cell_of(x, inv_cell, G);
-        const uint32_t slot = atomicAdd(&cstart[cc], 1u);
-        hole[slot] = make_float4(x, y, z, __uint_as_float(0xFFFF0000u | (uint32_t)h));
+    if (w0) {
+        for (int h = lane; h < n_h; h += kWave) {
+            const u32x4 o = philox10(0u, (uint32_t)h, prun, psim | MCL_TAG_GEOM_H, k0, k1);
+            const float x = unit_coord(o.x, R.side_b_nm), y = unit_coord(o.y, R.side_b_nm),
+                        z = unit_coord(o.z, R.side_b_nm);
+            const int cc = (cell_of(z, inv_cell, G) * G + cell_of(y, inv_cell, G)) * G +
+                           cell_of(x, inv_cell, G);
+            const uint32_t slot = atomicAdd(&cstart[cc], 1u);
+            hole[slot] = make_float4(x, y, z, __uint_as_float(0xFFFF0000u | (uint32_t)h));
+        }
     }
     __syncthreads();
     // cstart[c] now holds the end of cell c: shift right by one (top chunk first)
     for (int base = (n_cells / kWave) * kWave; base >= 0; base -= kWave) {
         const int i = base + lane;
-        const uint32_t v = (i >= 1 && i <= n_cells) ? cstart[i - 1] : 0u;
+        const uint32_t v = (w0 && i >= 1 && i <= n_cells) ? cstart[i - 1] : 0u;
         __syncthreads();
-        if (i <= n_cells) cstart[i] = v;
+        if (w0 && i <= n_cells) cstart[i] = v;
         __syncthreads();
     }
 
@@ -413,7 +424,7 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
 
     // ---- electrons and their initial nearest hole (Box._rebuild, engine.py:113-119)
     const int rows0 = (n_e0 + kWave - 1) / kWave;
-    for (int r = 0; r < rows0; ++r) {
+    for (int r = wv; r < rows0; r += NW) {
         const int e = r * kWave + lane;
         const bool act = e < n_e0;
         float x = 0.0f, y = 0.0f, z = 0.0f;
@@ -430,6 +441,15 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
     __syncthreads();
 
     return grid;
+}
+
+__device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGen& egen,
+                                              float4* hole, uint32_t* cstart, uint32_t* est,
+                                              float* ew, uint32_t prun, uint32_t psim,
+                                              uint32_t k0, uint32_t k1, int lane,
+                                              uint64_t& dist_evals)
+{
+    return setup_cluster_t<1>(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, lane, dist_evals);
 }
 
 // New nearest holes for the queued electrons rq[0..nq) ((slot << 16) | index):

@@ -1,8 +1,8 @@
-// mc_fixed_dt.hip — fixed-dt trap/charge-kinetics Monte Carlo, one wave64 per cluster.
+// mc_fixed_dt.hip — fixed-dt trap/charge-kinetics Monte Carlo, one workgroup per cluster.
 //
 // Replaces the run x sim x step loop of simulate.py:460-516 (Box / Physics /
-// TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of one
-// wavefront owns one cluster; everything the step touches lives in LDS:
+// TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of NW
+// wavefronts (1, 2 or 4) owns one cluster; everything the step touches lives in LDS:
 //
 //   hole       [h_cap]   16-byte records {x, y, z, meta}, float32 nm, sorted by
 //                        grid cell; meta = (claiming electron << 16) | original
@@ -13,6 +13,18 @@
 //   ethr       [e_cap]   fire threshold of a candidate in the current step (conflict
 //                        retries); the draw of an electron queued for a new search
 //   rq         [e_cap]   electrons whose nearest hole must be searched again: (slot << 16) | e
+//   xch        [2 NW + 1] (NW > 1) per-wave values exchanged between the waves, and the
+//                        persistent launch's ticket
+//
+// With NW > 1 the 64-slot rows of est/ew are split into NW contiguous ranges, one
+// per wave.  Every per-electron decision is the one-wave kernel's (same draws,
+// same thresholds, same lowest-index conflict rule, same stable compaction
+// order), so the counts are identical for every NW; the waves meet at workgroup
+// barriers where one wave's results reach the others: after the fire pass (any
+// candidate?), after each conflict round's winners (wins, any loser?), at each
+// retry round's claims, and in a compaction (per-wave keep counts; the rows are
+// rewritten into the spare est/ew pair, rq and ethr, whose roles then swap).
+// Each wave searches the queue of its own rows in its own part of rq.
 //
 // Electron coordinates are not stored: they are a pure function of the
 // Philox counter (0, e, run, sim|GEOM_E) and are regenerated when an
@@ -58,6 +70,8 @@
 // after the last electron recombines, like the reference's padded rows.
 #include "mc_common.h"
 
+#include <algorithm>
+
 namespace mcl {
 
 constexpr int kCachePairs = 4;    // row pairs whose Philox blocks stay in registers
@@ -66,11 +80,32 @@ struct CacheSlot {
     static constexpr int value = V;
 };
 
-// Simulate cluster c from its geometry to its last step, all state in lds_raw.
+// LDS ordering between the steps of one wave that write and then read the
+// search queue: the workgroup barrier with one wave (where it is also the
+// compiler's fence), a wavefront-scope fence with several (each wave owns its
+// queue; a workgroup barrier there would not be reached by every wave).
+template <int NW>
+__device__ __forceinline__ void queue_sync()
+{
+    if constexpr (NW == 1) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Simulate cluster c from its geometry to its last step, all state in lds_raw,
+// with NW wavefronts.  Every branch that holds a workgroup barrier depends only
+// on values all waves share (n, n_live, n_slots, and results of the exchanges).
+template <int NW>
 __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                             unsigned char* lds_raw)
 {
-    const int lane = (int)threadIdx.x;
+    const int tid = (int)threadIdx.x;
+    const int lane = NW == 1 ? tid : (tid & (kWave - 1));
+    const int wv = NW == 1 ? 0 : uniform(tid >> 6);
     const int run = uniform(a.c_run[c]);
     const int sim = uniform(a.c_sim[c]);
     const RunDesc R = a.runs[run];
@@ -87,6 +122,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     float* ew = (float*)(est + e_cap);
     uint32_t* ethr = (uint32_t*)(ew + e_cap);
     uint32_t* rq = ethr + e_cap;
+    uint32_t* const xch = rq + e_cap;
 
     const uint32_t k0 = a.key0, k1 = a.key1;
     const uint32_t prun = (uint32_t)R.run_id;
@@ -94,8 +130,8 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     const ElectronGen egen{prun, psim | MCL_TAG_GEOM_E, k0, k1, R.side_nm};
 
     uint64_t dist_evals = 0;
-    const Grid grid = setup_cluster(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, lane,
-                                    dist_evals);
+    const Grid grid = setup_cluster_t<NW>(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, tid,
+                                          dist_evals);
     const float alpha_f = (float)R.alpha_nm;
 
     // ---- step loop (simulate.py:475-516) -------------------------------------------
@@ -162,10 +198,33 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         cand |= 1ull << rr;
         return false;
     };
-    // Search again for the queued electrons rq[0..nq): one electron at a time
+    // Search again for the queued electrons q[0..nq): one electron at a time
     // with the whole wave while the queue is short, lane-parallel otherwise.
-    auto search_queue = [&](int nq) {
-        search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane, dist_evals);
+    auto search_queue = [&](const uint32_t* q, int nq) {
+        search_queue_nn(grid, egen, est, ew, q, nq, a.wave_search_max, alpha_f, lane, dist_evals);
+    };
+    // Per-wave values between the waves (NW > 1): lane 0 of each wave publishes
+    // v, one workgroup barrier, then every wave reads all NW values.  Consecutive
+    // exchanges alternate between two slots of xch and each holds a barrier, so a
+    // slot is rewritten only after every wave has read it.
+    int xph = 0;
+    auto exchange = [&](uint32_t v) -> const uint32_t* {
+        uint32_t* slot = xch + (xph & 1) * NW;
+        if (lane == 0) slot[wv] = v;
+        __syncthreads();
+        ++xph;
+        return slot;
+    };
+    auto any_wave = [&](bool v) -> bool {
+        if constexpr (NW == 1) {
+            return v;
+        } else {
+            const uint32_t* x = exchange(v ? 1u : 0u);
+            uint32_t r = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) r |= x[w];
+            return uniform(r) != 0u;
+        }
     };
 
     // Step scalars of step m: start-of-step temperature, Boltzmann factors in
@@ -226,7 +285,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         bin_rem += nb;
         if (bin_rem >= n_steps) {
             bin_rem -= n_steps;
-            if (lane == 0) cnt[bin] = bin_acc;
+            if (tid == 0) cnt[bin] = bin_acc;
             bin_acc = 0;
             ++bin;
         }
@@ -288,6 +347,15 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         const uint32_t blk = (uint32_t)n >> 2;
         const int word = n & 3;
         const int rows = (n_slots + kWave - 1) / kWave;
+        // this wave's rows: [r_lo, r_lo + nr), slots below slot_end (all rows with NW = 1)
+        int r_lo = 0, nr = rows, slot_end = n_slots;
+        if constexpr (NW > 1) {
+            const int per = (rows + NW - 1) / NW;
+            r_lo = min(rows, wv * per);
+            nr = min(rows, r_lo + per) - r_lo;
+            slot_end = min(n_slots, (r_lo + nr) * kWave);
+        }
+        uint32_t* const rq_w = rq + r_lo * kWave;     // this wave's part of the queue
         const float bmax = fmaxf(b1dt, b2dt);
 
         // Pass A, two rows per iteration: an electron whose nearest hole was removed
@@ -301,12 +369,12 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             cache_blk = (int)blk;
             cache_pairs = 0u;
         }
-        auto pass_a_pair = [&](int r, auto slot_tag) {
+        auto pass_a_pair = [&](int r, auto slot_tag) {      // r: the wave's row index
             constexpr int q = decltype(slot_tag)::value;   // register cache slot, -1: none
-            const int p0 = r * kWave + lane, p1 = p0 + kWave;
-            const uint32_t st0 = p0 < n_slots ? est[p0] : kDeadSlot;
-            const uint32_t st1 = p1 < n_slots ? est[p1] : kDeadSlot;
-            const float w0 = p0 < n_slots ? ew[p0] : 0.0f, w1 = p1 < n_slots ? ew[p1] : 0.0f;
+            const int p0 = (r_lo + r) * kWave + lane, p1 = p0 + kWave;
+            const uint32_t st0 = p0 < slot_end ? est[p0] : kDeadSlot;
+            const uint32_t st1 = p1 < slot_end ? est[p1] : kDeadSlot;
+            const float w0 = p0 < slot_end ? ew[p0] : 0.0f, w1 = p1 < slot_end ? ew[p1] : 0.0f;
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             // Below x = 2^-32 (in either channel) the threshold is 0: no draw can
             // fire, and a stale electron keeps waiting.  Rows whose lanes all sit
@@ -355,28 +423,28 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const bool s0 = test_fire(st0, o0, p0, u0, x0, r, cand);
             const bool s1 = test_fire(st1, o1, p1, u1, x1, r + 1, cand);
             const uint64_t m0 = ballot(s0);
-            if (s0) rq[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
+            if (s0) rq_w[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
             nq += __popcll(m0);
             const uint64_t m1 = ballot(s1);
-            if (s1) rq[nq + lanes_below(m1)] = ((uint32_t)p1 << 16) | o1;
+            if (s1) rq_w[nq + lanes_below(m1)] = ((uint32_t)p1 << 16) | o1;
             nq += __popcll(m1);
             stale |= ((uint64_t)s0 << r) | ((uint64_t)s1 << (r + 1));
             stale_ch |= ((uint64_t)ch0 << r) | ((uint64_t)ch1 << (r + 1));
         };
-        pass_a_pair(0, CacheSlot<0>{});
-        if (rows > 2) pass_a_pair(2, CacheSlot<1>{});
-        if (rows > 4) pass_a_pair(4, CacheSlot<2>{});
-        if (rows > 6) pass_a_pair(6, CacheSlot<3>{});
-        for (int r = 2 * kCachePairs; r < rows; r += 2) pass_a_pair(r, CacheSlot<-1>{});
+        if (NW == 1 || nr > 0) pass_a_pair(0, CacheSlot<0>{});
+        if (nr > 2) pass_a_pair(2, CacheSlot<1>{});
+        if (nr > 4) pass_a_pair(4, CacheSlot<2>{});
+        if (nr > 6) pass_a_pair(6, CacheSlot<3>{});
+        for (int r = 2 * kCachePairs; r < nr; r += 2) pass_a_pair(r, CacheSlot<-1>{});
         if (nq > 0) {
-            __syncthreads();
-            search_queue(nq);
-            __syncthreads();
-            uint64_t m = rows_any(stale, rows);
+            queue_sync<NW>();
+            search_queue(rq_w, nq);
+            queue_sync<NW>();
+            uint64_t m = rows_any(stale, nr);
             while (m != 0) {
                 const int r = __builtin_ctzll(m);
                 m &= m - 1;
-                const int p = r * kWave + lane;
+                const int p = (r_lo + r) * kWave + lane;
                 if ((stale >> r) & 1ull) {
                     // the draw and channel of pass A, the new w
                     const uint32_t st = est[p];
@@ -386,9 +454,10 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                 }
             }
         }
-        uint64_t rowmask = rows_any(cand, rows);
-        uint32_t wins = 0;
-        if (rowmask != 0) {
+        uint64_t rowmask = rows_any(cand, nr);
+        uint32_t wins = 0;          // this wave's winners in this step
+        uint32_t step_wins = 0;     // the cluster's
+        if (any_wave(rowmask != 0)) {
             for (int round = 0;; ++round) {
                 if (round > 0) {
                     // claims of this round's candidates (losers accepted for a retry)
@@ -397,19 +466,22 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                         const int r = __builtin_ctzll(m);
                         m &= m - 1;
                         if ((cand >> r) & 1ull) {
-                            const uint32_t st = est[r * kWave + lane];
+                            const uint32_t st = est[(r_lo + r) * kWave + lane];
                             claim(st >> 16, st & 0xFFFFu);
                         }
                     }
+                    if constexpr (NW > 1) {
+                        if (!any_wave(rowmask != 0)) break;   // also the claims' barrier
+                    }
                 }
-                __syncthreads();
+                if constexpr (NW == 1) __syncthreads();
                 // winners: the claim field holds the lowest candidate index for that hole
                 uint64_t lose = 0;
                 uint64_t m = rowmask;
                 while (m != 0) {
                     const int r = __builtin_ctzll(m);
                     m &= m - 1;
-                    const int p = r * kWave + lane;
+                    const int p = (r_lo + r) * kWave + lane;
                     const bool is_cand = (cand >> r) & 1ull;
                     bool win = false;
                     uint32_t nn = 0, meta = 0;
@@ -427,10 +499,26 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                     }
                     wins += (uint32_t)__popcll(ballot(win));
                 }
-                __syncthreads();
-                if (round + 1 == MCL_MAX_ROUNDS) break;
-                const uint64_t lrows = rows_any(lose, rows);
-                if (lrows == 0) break;
+                uint64_t lrows;
+                if constexpr (NW == 1) {
+                    __syncthreads();
+                    if (round + 1 == MCL_MAX_ROUNDS) break;
+                    lrows = rows_any(lose, rows);
+                    if (lrows == 0) break;
+                } else {
+                    // the removed holes are visible to every wave after this barrier
+                    lrows = rows_any(lose, nr);
+                    const uint32_t* x = exchange(wins | (lrows != 0 ? 0x80000000u : 0u));
+                    uint32_t t = 0, l = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        t += x[w] & 0x7FFFFFFFu;
+                        l |= x[w] >> 31;
+                    }
+                    step_wins = uniform(t);
+                    if (round + 1 == MCL_MAX_ROUNDS) break;
+                    if (uniform(l) == 0u) break;
+                }
 
                 // a loser's hole is gone: search again now, then it retries in the same
                 // step, accepted with probability T_new / T_old (thinning of the draw
@@ -441,22 +529,22 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                 while (m != 0) {
                     const int r = __builtin_ctzll(m);
                     m &= m - 1;
-                    const int p = r * kWave + lane;
+                    const int p = (r_lo + r) * kWave + lane;
                     const bool l = (lose >> r) & 1ull;
                     const uint64_t mm = ballot(l);
-                    if (l) rq[nl + lanes_below(mm)] = ((uint32_t)p << 16) | (est[p] & 0xFFFFu);
+                    if (l) rq_w[nl + lanes_below(mm)] = ((uint32_t)p << 16) | (est[p] & 0xFFFFu);
                     nl += __popcll(mm);
                 }
-                __syncthreads();
-                search_queue(nl);
-                __syncthreads();
+                queue_sync<NW>();
+                search_queue(rq_w, nl);
+                queue_sync<NW>();
                 cand = 0;
                 const uint32_t c3r = psim | (MCL_TAG_RETRY0 + (uint32_t)round);
                 m = lrows;
                 while (m != 0) {
                     const int r = __builtin_ctzll(m);
                     m &= m - 1;
-                    const int p = r * kWave + lane;
+                    const int p = (r_lo + r) * kWave + lane;
                     if ((lose >> r) & 1ull) {
                         const uint32_t st = est[p];
                         const uint32_t o = st & 0xFFFFu, nn = st >> 16;
@@ -484,44 +572,87 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                         }
                     }
                 }
-                rowmask = rows_any(cand, rows);
-                if (rowmask == 0) break;
+                rowmask = rows_any(cand, nr);
+                if constexpr (NW == 1) {
+                    if (rowmask == 0) break;
+                }
             }
-            n_live -= (int)wins;
+            if constexpr (NW == 1) step_wins = wins;
+            n_live -= (int)step_wins;
 
             // lazy stable compaction when a 64-slot row can be dropped (an electron
             // waiting for a new search moves with its stale nearest hole)
             if ((n_live + kWave - 1) / kWave < rows) {
-                int base = 0;
-                for (int r = 0; r < rows; ++r) {
-                    const int p = r * kWave + lane;
-                    const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
-                    const float w = p < n_slots ? ew[p] : 0.0f;
-                    const bool keep = st != kDeadSlot;
-                    const uint64_t km = ballot(keep);
-                    if (keep) {
-                        const int q = base + lanes_below(km);
-                        est[q] = st;
-                        ew[q] = w;
+                if constexpr (NW == 1) {
+                    int base = 0;
+                    for (int r = 0; r < rows; ++r) {
+                        const int p = r * kWave + lane;
+                        const uint32_t st = p < n_slots ? est[p] : kDeadSlot;
+                        const float w = p < n_slots ? ew[p] : 0.0f;
+                        const bool keep = st != kDeadSlot;
+                        const uint64_t km = ballot(keep);
+                        if (keep) {
+                            const int q = base + lanes_below(km);
+                            est[q] = st;
+                            ew[q] = w;
+                        }
+                        base += __popcll(km);
                     }
-                    base += __popcll(km);
+                    n_slots = base;
+                } else {
+                    // per-wave keep counts give each wave's first new slot; the rows
+                    // are rewritten into the spare pair (rq, ethr), which becomes est/ew
+                    uint32_t kept = 0;
+                    for (int r = 0; r < nr; ++r) {
+                        const int p = (r_lo + r) * kWave + lane;
+                        const uint32_t st = p < slot_end ? est[p] : kDeadSlot;
+                        kept += (uint32_t)__popcll(ballot(st != kDeadSlot));
+                    }
+                    const uint32_t* x = exchange(kept);
+                    uint32_t before = 0, total = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        before += w < wv ? x[w] : 0u;
+                        total += x[w];
+                    }
+                    int base = (int)uniform(before);
+                    uint32_t* const n_est = rq;
+                    float* const n_ew = (float*)ethr;
+                    for (int r = 0; r < nr; ++r) {
+                        const int p = (r_lo + r) * kWave + lane;
+                        const uint32_t st = p < slot_end ? est[p] : kDeadSlot;
+                        const float w = p < slot_end ? ew[p] : 0.0f;
+                        const bool keep = st != kDeadSlot;
+                        const uint64_t km = ballot(keep);
+                        if (keep) {
+                            const int q = base + lanes_below(km);
+                            n_est[q] = st;
+                            n_ew[q] = w;
+                        }
+                        base += __popcll(km);
+                    }
+                    rq = est;
+                    ethr = (uint32_t*)ew;
+                    est = n_est;
+                    ew = n_ew;
+                    n_slots = (int)uniform(total);
                 }
-                n_slots = base;
                 cache_pairs = 0u;                   // electrons moved to other slots
                 __syncthreads();
             }
         }
-        finish_step(wins);
+        finish_step(step_wins);
         ++n;
     }
     if (bin < nb) {
-        if (lane == 0) cnt[bin] = bin_acc;
+        if (tid == 0) cnt[bin] = bin_acc;
         ++bin;
     }
-    for (int b = bin + lane; b < a.row_len; b += kWave) cnt[b] = 0u;
-    if (a.live_steps != nullptr && lane == 0) a.live_steps[c] = lsteps;
+    for (int b = bin + tid; b < a.row_len; b += NW * kWave) cnt[b] = 0u;
+    if (a.live_steps != nullptr && tid == 0) a.live_steps[c] = lsteps;
 
-    if (a.dump.enabled) {
+    // (state dumps are one-wave launches only)
+    if (NW == 1 && a.dump.enabled) {
         // searches deferred to the next step happen now, so the dump shows every
         // survivor's current nearest hole
         __syncthreads();
@@ -535,7 +666,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             nq += __popcll(m);
         }
         __syncthreads();
-        search_queue(nq);
+        search_queue(rq, nq);
         __syncthreads();
     }
 
@@ -547,14 +678,25 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(evals_total >> 32), off);
         evals_total += ((uint64_t)hi << 32) | lo;
     }
-    if (a.work != nullptr && lane == 0) {
+    if constexpr (NW > 1) {
+        const uint32_t* xl = exchange((uint32_t)evals_total);
+        uint64_t lo_sum = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) lo_sum += xl[w];
+        const uint32_t* xh = exchange((uint32_t)(evals_total >> 32));
+        uint64_t hi_sum = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) hi_sum += xh[w];
+        evals_total = lo_sum + (hi_sum << 32);
+    }
+    if (a.work != nullptr && tid == 0) {
         a.work[4 * (size_t)c] = lsteps;
         a.work[4 * (size_t)c + 1] = evals_total;
         a.work[4 * (size_t)c + 2] = (uint64_t)(n_e0 - n_live);
         a.work[4 * (size_t)c + 3] = (uint64_t)steps_run;
     }
 
-    if (a.dump.enabled) {
+    if (NW == 1 && a.dump.enabled) {
         const StateDump& d = a.dump;
         for (int e = lane; e < n_e0; e += kWave) {
             float x, y, z;
@@ -597,7 +739,7 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (a.queue == nullptr) {
         const int i = blockIdx.x;
-        if (i < a.n_order) run_cluster(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        if (i < a.n_order) run_cluster<1>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         return;
     }
     for (;;) {
@@ -605,7 +747,33 @@ __global__ __launch_bounds__(64) void fixed_dt_kernel(LaunchArgs a)
         if (threadIdx.x == 0) i = (int)atomicAdd(a.queue, 1u);
         i = __shfl(i, 0);
         if (i >= a.n_order) break;
-        run_cluster(a, a.order[i], lds_raw);
+        run_cluster<1>(a, a.order[i], lds_raw);
+        __syncthreads();
+    }
+}
+
+// The same with NW wavefronts per cluster (at least two waves per SIMD, so at
+// most 256 VGPRs a wave, as the one-wave kernel).  The persistent ticket goes
+// through LDS, read by every wave between two barriers, so all waves take the
+// same cluster and leave the loop together.
+template <int NW>
+__global__ __launch_bounds__(NW * 64, 2) void fixed_dt_mw_kernel(LaunchArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if (a.queue == nullptr) {
+        const int i = blockIdx.x;
+        if (i < a.n_order) run_cluster<NW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        return;
+    }
+    uint32_t* const ticket =
+        (uint32_t*)(lds_raw + 16 * a.h_cap + 4 * a.g_cap + 16 * a.e_cap) + 2 * NW;
+    for (;;) {
+        if (threadIdx.x == 0) *ticket = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const int i = (int)uniform(*ticket);
+        __syncthreads();
+        if (i >= a.n_order) break;
+        run_cluster<NW>(a, a.order[i], lds_raw);
         __syncthreads();
     }
 }
@@ -618,16 +786,37 @@ __global__ void reset_tickets_kernel(uint32_t* tickets, int n)
     if (i < n) tickets[i] = 0u;
 }
 
-hipError_t prepare_fixed_dt(int lds_bytes)
+namespace {
+const void* fixed_dt_fn(int waves)
 {
-    return hipFuncSetAttribute((const void*)fixed_dt_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    return waves == 4 ? (const void*)fixed_dt_mw_kernel<4>
+         : waves == 2 ? (const void*)fixed_dt_mw_kernel<2>
+                      : (const void*)fixed_dt_kernel;
+}
+}  // namespace
+
+hipError_t prepare_fixed_dt(int lds_bytes, int max_lds)
+{
+    hipError_t e = hipSuccess;
+    for (int w : {1, 2, 4}) {
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(fixed_dt_fn(w), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    std::min(lds_bytes + fixed_dt_waves_lds(w), max_lds));
+    }
+    return e;
 }
 
-hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, hipStream_t stream)
+hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int waves,
+                           hipStream_t stream)
 {
-    hipLaunchKernelGGL(fixed_dt_kernel, dim3((unsigned)grid), dim3(kWave), (unsigned)lds_bytes,
-                       stream, a);
+    const dim3 g((unsigned)grid);
+    const unsigned lds = (unsigned)(lds_bytes + fixed_dt_waves_lds(waves));
+    if (waves == 4)
+        hipLaunchKernelGGL(fixed_dt_mw_kernel<4>, g, dim3(4 * kWave), lds, stream, a);
+    else if (waves == 2)
+        hipLaunchKernelGGL(fixed_dt_mw_kernel<2>, g, dim3(2 * kWave), lds, stream, a);
+    else
+        hipLaunchKernelGGL(fixed_dt_kernel, g, dim3(kWave), lds, stream, a);
     return hipGetLastError();
 }
 
@@ -638,10 +827,11 @@ hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu)
+hipError_t fixed_dt_occupancy(int lds_bytes, int waves, int* blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)fixed_dt_kernel,
-                                                        kWave, (size_t)lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, fixed_dt_fn(waves), waves * kWave,
+        (size_t)(lds_bytes + fixed_dt_waves_lds(waves)));
 }
 
 }  // namespace mcl

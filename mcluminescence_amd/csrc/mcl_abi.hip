@@ -38,6 +38,7 @@ struct Plan {
     int32_t row_len = 0;          // counts row stride
     int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0;
     int32_t max_e = 0, max_h = 0;
+    int32_t waves = 1;            // wavefronts per cluster of the one-workgroup-per-cluster launch
     uint32_t flags = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -46,6 +47,7 @@ struct Plan {
         int32_t first = 0, count = 0;   // slice of d_order (clusters, most work first)
         int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0, grid = 0;
         int32_t max_e = 0, max_h = 0;
+        int32_t waves = 1;
     };
     std::vector<Class> classes;
     int32_t* d_order = nullptr;
@@ -178,6 +180,35 @@ void plan_free(Plan* p)
     delete p;
 }
 
+// Wavefronts per cluster for `count` clusters of at most e_cap electrons that
+// need lds bytes each: the tuning knob when set, otherwise the one of 1, 2, 4
+// that keeps the most waves resident (clusters resident at once, from the
+// occupancy of that kernel, times waves), the fewest on a tie; a wave gets at
+// least one 64-slot row at the start.  per_cu: that kernel's workgroups per CU.
+int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int* waves, int* per_cu)
+{
+    const int forced = (int)c->waves_per_cluster;
+    int best = 1, best_pc = 0;
+    int64_t best_res = -1;
+    for (int w : {1, 2, 4}) {
+        if (forced != 0 && w != forced) continue;
+        if (forced == 0 && w > 1 && w > e_cap / 64) break;   // 64-slot rows
+        if (lds + mcl::fixed_dt_waves_lds(w) > c->max_lds) continue;
+        int pc = 0;
+        const hipError_t e = mcl::fixed_dt_occupancy(lds, w, &pc);
+        if (e != hipSuccess) return fail(c, MCL_EDEVICE, "occupancy query: %s", hipGetErrorString(e));
+        const int64_t res = (int64_t)w * std::min<int64_t>(count, (int64_t)std::max(pc, 1) * c->n_cu);
+        if (res > best_res) {
+            best = w;
+            best_pc = pc;
+            best_res = res;
+        }
+    }
+    *waves = best;
+    *per_cu = best_pc;
+    return MCL_OK;
+}
+
 // Persistent plans: bucket the clusters by electron count (powers of two from
 // 128) so each class kernel sizes its LDS for its own clusters, order each class
 // by expected work (electrons x steps, largest first) and give it as many
@@ -229,10 +260,10 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
         if (cl.lds > c->max_lds)
             return fail(c, MCL_ERANGE, "cluster of %d electrons / %d holes needs %d B of LDS; a "
                         "workgroup has %d B", cl.max_e, cl.max_h, cl.lds, c->max_lds);
-        // (the kernel's dynamic-LDS limit was raised to the plan's largest class)
+        // (the kernels' dynamic-LDS limit was raised to the plan's largest class)
         int per_cu = 0;
-        const hipError_t e = mcl::fixed_dt_occupancy(cl.lds, &per_cu);
-        if (e != hipSuccess) return fail(c, MCL_EDEVICE, "occupancy query: %s", hipGetErrorString(e));
+        const int st = choose_waves(c, cl.lds, cl.e_cap, cl.count, &cl.waves, &per_cu);
+        if (st) return st;
         const double frac = c->persist_frac;
         int grid = (int)std::max(1.0, std::floor((double)std::max(per_cu, 1) * c->n_cu * frac));
         cl.grid = std::min(grid, cl.count);
@@ -329,11 +360,19 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipEventCreate(&p->ev0);
     if (e == hipSuccess) e = hipEventCreate(&p->ev1);
-    if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds);
+    if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds, c->max_lds);
     if (e != hipSuccess) {
         plan_free(p);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "plan setup: %s",
                     hipGetErrorString(e));
+    }
+    {
+        int per_cu = 0;
+        const int st = choose_waves(c, lds, e_cap, total, &p->waves, &per_cu);
+        if (st) {
+            plan_free(p);
+            return st;
+        }
     }
     if (flags & (MCL_F_PERSISTENT | MCL_F_GRAPH)) {
         const int st = build_classes(c, p, runs, n_runs, desc);
@@ -390,7 +429,7 @@ hipError_t enqueue_persistent(Plan* p, uint32_t* counts, uint64_t* live, hipStre
         a.h_cap = cl.h_cap;
         a.e_cap = cl.e_cap;
         a.g_cap = cl.g_cap;
-        if (e == hipSuccess) e = mcl::launch_fixed_dt(a, cl.grid, cl.lds, sk);
+        if (e == hipSuccess) e = mcl::launch_fixed_dt(a, cl.grid, cl.lds, cl.waves, sk);
         if (e == hipSuccess && k > 0) e = hipEventRecord(p->join[(size_t)k], sk);
         if (e == hipSuccess && k > 0) e = hipStreamWaitEvent(s, p->join[(size_t)k], 0);
     }
@@ -438,7 +477,8 @@ int plan_launch(Plan* p, uint32_t* counts, uint64_t* live, const mcl::StateDump*
     mcl::LaunchArgs a = launch_args(p, counts, live);
     if (dump) a.dump = *dump;
     HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
-    HIP_TRY(c, mcl::launch_fixed_dt(a, (int)p->total, p->lds, c->stream));
+    // (state dumps run one wave per cluster)
+    HIP_TRY(c, mcl::launch_fixed_dt(a, (int)p->total, p->lds, dump ? 1 : p->waves, c->stream));
     HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
     p->timed = true;
     return MCL_OK;
@@ -540,6 +580,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->persist_frac = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "waves_per_cluster") == 0) {
+        if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0))
+            return fail(c, MCL_EINVAL, "waves_per_cluster must be 0, 1, 2 or 4 (got %g)", value);
+        c->waves_per_cluster = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -561,6 +607,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "wave_search_max") == 0) {
         *value = c->wave_search_max;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "waves_per_cluster") == 0) {
+        *value = c->waves_per_cluster;
         return MCL_OK;
     }
     return MCL_EINVAL;

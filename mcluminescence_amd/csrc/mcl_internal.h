@@ -125,10 +125,20 @@ inline int grid_dim(int n_h, double holes_per_cell)
     return G;
 }
 
-hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, hipStream_t stream);
+// LDS beyond fixed_dt_lds_bytes for a workgroup of `waves` wavefronts: the
+// per-wave exchange slots (2 x waves words) and the persistent ticket.
+inline int fixed_dt_waves_lds(int waves)
+{
+    return waves > 1 ? ((4 * (2 * waves + 1) + 15) / 16) * 16 : 0;
+}
+
+// waves = wavefronts per cluster (1, 2 or 4); lds_bytes = fixed_dt_lds_bytes of the
+// launch (the per-wave extra is added here)
+hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int waves,
+                           hipStream_t stream);
 hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream);
-hipError_t prepare_fixed_dt(int lds_bytes);
-hipError_t fixed_dt_occupancy(int lds_bytes, int* blocks_per_cu);
+hipError_t prepare_fixed_dt(int lds_bytes, int max_lds);
+hipError_t fixed_dt_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
 hipError_t launch_lab(const LabArgs& a, int lds_bytes, hipStream_t stream);
 hipError_t prepare_lab(int lds_bytes);
