@@ -118,7 +118,7 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *   "lookahead_max"   with at most this many live electrons in a cluster, the
  *                     wave tests the next 64 steps at once and skips those in
  *                     which no electron can fire (default 8; 0 turns it off).
- *   "waves_per_cluster" wavefronts per fixed-dt cluster: 1, 2 or 4 (the waves
+ *   "waves_per_cluster" wavefronts per fixed-dt cluster: 1, 2, 4 or 8 (the waves
  *                     split the cluster's 64-electron rows), or 0 (default):
  *                     each launch, or each size class of a persistent launch,
  *                     takes the count that keeps the most waves resident.

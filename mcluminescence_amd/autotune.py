@@ -12,7 +12,7 @@ The kernels' results do not depend on the tuning knobs (mcl_tune_set):
   persist_frac     persistent workgroups per size class, as a fraction of what
                    fits on the device at once (load balance against contention
                    between concurrently running class kernels)
-  waves_per_cluster  wavefronts per cluster (1, 2, 4; 0 lets each launch or size
+  waves_per_cluster  wavefronts per cluster (1, 2, 4, 8; 0 lets each launch or size
                    class pick the count that keeps the most waves resident)
 
 so the tuner times every candidate on the workload itself (a plan launch on

@@ -2,7 +2,7 @@
 //
 // Replaces the run x sim x step loop of simulate.py:460-516 (Box / Physics /
 // TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of NW
-// wavefronts (1, 2 or 4) owns one cluster; everything the step touches lives in LDS:
+// wavefronts (1, 2, 4 or 8) owns one cluster; everything the step touches lives in LDS:
 //
 //   hole       [h_cap]   16-byte records {x, y, z, meta}, float32 nm, sorted by
 //                        grid cell; meta = (claiming electron << 16) | original
@@ -789,7 +789,8 @@ __global__ void reset_tickets_kernel(uint32_t* tickets, int n)
 namespace {
 const void* fixed_dt_fn(int waves)
 {
-    return waves == 4 ? (const void*)fixed_dt_mw_kernel<4>
+    return waves == 8 ? (const void*)fixed_dt_mw_kernel<8>
+         : waves == 4 ? (const void*)fixed_dt_mw_kernel<4>
          : waves == 2 ? (const void*)fixed_dt_mw_kernel<2>
                       : (const void*)fixed_dt_kernel;
 }
@@ -798,7 +799,7 @@ const void* fixed_dt_fn(int waves)
 hipError_t prepare_fixed_dt(int lds_bytes, int max_lds)
 {
     hipError_t e = hipSuccess;
-    for (int w : {1, 2, 4}) {
+    for (int w : {1, 2, 4, 8}) {
         if (e == hipSuccess)
             e = hipFuncSetAttribute(fixed_dt_fn(w), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     std::min(lds_bytes + fixed_dt_waves_lds(w), max_lds));
@@ -811,7 +812,9 @@ hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int wav
 {
     const dim3 g((unsigned)grid);
     const unsigned lds = (unsigned)(lds_bytes + fixed_dt_waves_lds(waves));
-    if (waves == 4)
+    if (waves == 8)
+        hipLaunchKernelGGL(fixed_dt_mw_kernel<8>, g, dim3(8 * kWave), lds, stream, a);
+    else if (waves == 4)
         hipLaunchKernelGGL(fixed_dt_mw_kernel<4>, g, dim3(4 * kWave), lds, stream, a);
     else if (waves == 2)
         hipLaunchKernelGGL(fixed_dt_mw_kernel<2>, g, dim3(2 * kWave), lds, stream, a);

@@ -181,18 +181,24 @@ void plan_free(Plan* p)
 }
 
 // Wavefronts per cluster for `count` clusters of at most e_cap electrons that
-// need lds bytes each: the tuning knob when set, otherwise the one of 1, 2, 4
-// that keeps the most waves resident (clusters resident at once, from the
-// occupancy of that kernel, times waves), the fewest on a tie; a wave gets at
-// least one 64-slot row at the start.  per_cu: that kernel's workgroups per CU.
-int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int* waves, int* per_cu)
+// need lds bytes each: the tuning knob when set, otherwise the one of 1, 2, 4, 8
+// (at most max_waves) that keeps the most waves resident (clusters resident at
+// once, from the occupancy of that kernel, times waves), the fewest on a tie.  A
+// wave gets at least one 64-slot row at the start, and eight waves need two rows
+// each: with one, four waves measured as fast.  Size classes that run side by side
+// take at most four, so that a class of large clusters leaves wave slots on its
+// CUs to the others (eight measured 20 % slower on an uneven sweep).  per_cu:
+// that kernel's workgroups per CU.
+int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int max_waves, int* waves,
+                 int* per_cu)
 {
     const int forced = (int)c->waves_per_cluster;
+    const int rows = e_cap / 64;                            // 64-slot rows
     int best = 1, best_pc = 0;
     int64_t best_res = -1;
-    for (int w : {1, 2, 4}) {
+    for (int w : {1, 2, 4, 8}) {
         if (forced != 0 && w != forced) continue;
-        if (forced == 0 && w > 1 && w > e_cap / 64) break;   // 64-slot rows
+        if (forced == 0 && (w > max_waves || w > rows || (w == 8 && rows < 16))) break;
         if (lds + mcl::fixed_dt_waves_lds(w) > c->max_lds) continue;
         int pc = 0;
         const hipError_t e = mcl::fixed_dt_occupancy(lds, w, &pc);
@@ -262,7 +268,8 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
                         "workgroup has %d B", cl.max_e, cl.max_h, cl.lds, c->max_lds);
         // (the kernels' dynamic-LDS limit was raised to the plan's largest class)
         int per_cu = 0;
-        const int st = choose_waves(c, cl.lds, cl.e_cap, cl.count, &cl.waves, &per_cu);
+        const int st = choose_waves(c, cl.lds, cl.e_cap, cl.count,
+                                    p->classes.size() > 1 ? 4 : 8, &cl.waves, &per_cu);
         if (st) return st;
         const double frac = c->persist_frac;
         int grid = (int)std::max(1.0, std::floor((double)std::max(per_cu, 1) * c->n_cu * frac));
@@ -368,7 +375,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     }
     {
         int per_cu = 0;
-        const int st = choose_waves(c, lds, e_cap, total, &p->waves, &per_cu);
+        const int st = choose_waves(c, lds, e_cap, total, 8, &p->waves, &per_cu);
         if (st) {
             plan_free(p);
             return st;
@@ -581,8 +588,8 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         return MCL_OK;
     }
     if (std::strcmp(key, "waves_per_cluster") == 0) {
-        if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0))
-            return fail(c, MCL_EINVAL, "waves_per_cluster must be 0, 1, 2 or 4 (got %g)", value);
+        if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0 || value == 8.0))
+            return fail(c, MCL_EINVAL, "waves_per_cluster must be 0, 1, 2, 4 or 8 (got %g)", value);
         c->waves_per_cluster = value;
         return MCL_OK;
     }

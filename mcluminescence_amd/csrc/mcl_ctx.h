@@ -22,7 +22,7 @@ struct Ctx {
     double persist_frac = 1.0;     // persistent workgroups per class / device capacity
     double wave_search_max = 32.0; // queue length searched by the whole wave
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
-    double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4 or 0 (chosen per launch)
+    double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
     std::string err;
 };
 

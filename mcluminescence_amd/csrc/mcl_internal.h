@@ -132,7 +132,7 @@ inline int fixed_dt_waves_lds(int waves)
     return waves > 1 ? ((4 * (2 * waves + 1) + 15) / 16) * 16 : 0;
 }
 
-// waves = wavefronts per cluster (1, 2 or 4); lds_bytes = fixed_dt_lds_bytes of the
+// waves = wavefronts per cluster (1, 2, 4 or 8); lds_bytes = fixed_dt_lds_bytes of the
 // launch (the per-wave extra is added here)
 hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int waves,
                            hipStream_t stream);

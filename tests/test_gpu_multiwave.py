@@ -1,4 +1,4 @@
-"""Several wavefronts per cluster: the 2- and 4-wave kernels give exactly the counts
+"""Several wavefronts per cluster: the 2-, 4- and 8-wave kernels give exactly the counts
 and live electron-steps of the one-wave kernel and of the CPU replay harness, on
 one-workgroup-per-cluster, persistent and graph launches, and at the extremes of
 the other tuning knobs.
@@ -49,7 +49,7 @@ def test_waves_give_identical_counts(gpu_ctx, name):
     runs, n_bins = _cases()[name]
     print(f"multiwave {name}: {sum(r.sims for r in runs)} clusters", flush=True)
     base, lb, _ = _run(gpu_ctx, 1, runs, n_bins=n_bins)
-    for w in (2, 4):
+    for w in (2, 4, 8):
         print(f"  {w} waves", flush=True)
         got, lg, _ = _run(gpu_ctx, w, runs, n_bins=n_bins)
         np.testing.assert_array_equal(got, base, err_msg=f"{name}: {w} waves")
@@ -75,7 +75,7 @@ def test_knob_extremes_with_waves(gpu_ctx):
         for la, wsm in [(0.0, 0.0), (65535.0, 64.0), (8.0, 2.0)]:
             gpu_ctx.tune_set("lookahead_max", la)
             gpu_ctx.tune_set("wave_search_max", wsm)
-            for w in (2, 4):
+            for w in (2, 4, 8):
                 print(f"  lookahead_max {la:g} wave_search_max {wsm:g}: {w} waves", flush=True)
                 got, lg, _ = _run(gpu_ctx, w, runs, n_bins=48)
                 np.testing.assert_array_equal(got, base)
@@ -90,7 +90,7 @@ def test_persistent_and_graph_launches_with_waves(gpu_ctx):
     runs = W.uneven_sweep(n_sets=240, n_steps=192, seed=11)
     print("multiwave persistent/graph: 240 uneven sets", flush=True)
     base, lb, _ = _run(gpu_ctx, 1, runs, n_bins=48)
-    for w in (0, 2, 4):                      # 0: each size class picks its own
+    for w in (0, 2, 4, 8):                   # 0: each size class picks its own
         print(f"  persistent, waves_per_cluster {w}", flush=True)
         got, lg, _ = _run(gpu_ctx, w, runs, n_bins=48, flags=F_PERSISTENT)
         np.testing.assert_array_equal(got, base)
@@ -111,7 +111,7 @@ def test_persistent_and_graph_launches_with_waves(gpu_ctx):
 
 
 def test_waves_knob_is_validated(gpu_ctx):
-    for bad in (3.0, -1.0, 8.0):
+    for bad in (3.0, -1.0, 16.0):
         with pytest.raises(MclError):
             gpu_ctx.tune_set("waves_per_cluster", bad)
     assert gpu_ctx.tune_get("waves_per_cluster") == 0.0
