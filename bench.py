@@ -19,7 +19,9 @@ roofline   = the MC kernel against the VALU issue rate of the chip: algorithmic
 cpu_baseline = the numpy event-mode restatement of the reference loop
              (simulate.py:460-516, one recombination per step, dense distance
              matrix) on a bounded sample of the same clusters, one core,
-             expressed in the same unit (fixed-dt-equivalent live electron-steps).
+             expressed in the same unit (fixed-dt-equivalent live electron-steps);
+             with the extras also on every host core (one process per cluster)
+             and the C fixed-dt replay harness on one and on every core.
 """
 from __future__ import annotations
 
@@ -66,6 +68,54 @@ def cpu_baseline(seconds: float) -> dict:
             "sample": f"{done} of the 4096 clusters (400 electrons, 2000 s at 500 K) in "
                       f"{el:.1f} s, numpy event-mode restatement of simulate.py:460-516",
             "curves_per_s": done / el}
+
+
+def _port_cluster(k: int) -> float:
+    """Live electron-steps of decay cluster k by the numpy port (a pool worker)."""
+    from oracle import event_mc as E
+    from mcluminescence_amd import workloads as W
+    run = W.iso_decay_4096x400(sims=1)[0]
+    exp, phys = E.run_params_to_cfg(run)
+    n0 = int(run.N_e * run.e_ratio_start)
+    x, lum, _ = E.simulate_run(exp, phys, sims=1, steps=n0 + 16, seed=0xC0FFEE, first_sim=k)
+    return float(E.live_electron_steps(x, lum, n0, run.dt, run.n_steps).sum())
+
+
+def _port_warm(_: int) -> int:
+    from oracle import event_mc  # noqa: F401
+    from mcluminescence_amd import workloads  # noqa: F401
+    return 0
+
+
+def cpu_baseline_pool(seconds: float) -> dict:
+    """The numpy port on every host core (at most 16): one process per cluster,
+    OMP_NUM_THREADS=1, clusters handed out until ~seconds have elapsed.  Called
+    before the process touches the GPU, so the forked workers carry no GPU state."""
+    import multiprocessing as mp
+    cores = min(16, os.cpu_count() or 1)
+    old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        esteps, done = 0.0, 0
+        with mp.get_context("fork").Pool(cores) as pool:
+            pool.map(_port_warm, range(cores))        # imports, outside the timed region
+            t0 = time.perf_counter()
+            el = 0.0
+            for v in pool.imap_unordered(_port_cluster, range(4096)):
+                esteps += v
+                done += 1
+                el = time.perf_counter() - t0
+                if el >= seconds:
+                    break
+            pool.terminate()
+    finally:
+        if old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = old
+    return {"value": esteps / el, "unit": "electron-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of the 4096 decay clusters in {el:.1f} s, numpy event-mode "
+                      f"restatement of simulate.py:460-516, one process per cluster"}
 
 
 def cpu_event_lab(seconds: float) -> dict:
@@ -429,6 +479,10 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    pool_leg = None
+    if not args.no_cpu and not args.no_extras:
+        pool_leg = cpu_baseline_pool(args.cpu_seconds / 2)
+
     import torch
     from mcluminescence_amd import autotune
     from mcluminescence_amd import workloads as W
@@ -521,6 +575,7 @@ def main() -> int:
         if not args.no_extras:
             result["cpu_baseline"].update(cpu_event_lab(args.cpu_seconds / 3))
             result["cpu_baseline"]["harness"] = cpu_harness(args.cpu_seconds / 3)
+            result["cpu_baseline"]["all_cores"] = pool_leg
         result["cpu_baseline"]["host_cpu"] = _cpu_model()
         result["cpu_baseline"]["os_cpu_count"] = os.cpu_count()
     print(json.dumps(result), flush=True)
