@@ -338,6 +338,16 @@ __device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_
     w = none ? 0.0f : w_of(d2bits, alpha_f);
 }
 
+// Orders the LDS accesses of one wave around it, for a workgroup of one wave: a
+// wavefront-scope fence for the compiler; the wave's LDS instructions execute in
+// issue order.
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Bit r set iff any lane has bit r of its row mask set (rows <= 64), by ballots.
 __device__ __forceinline__ uint64_t rows_any(uint64_t bits, int rows)
 {

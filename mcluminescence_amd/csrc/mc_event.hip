@@ -120,7 +120,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
         }
-        __syncthreads();
+        wave_lds_fence();
         const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
         const double lam = (R.D == 0.0 || n_live + filled == N_int)
                                ? 1e-20
@@ -162,42 +162,35 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             pw = uniform(pw);
             const uint32_t sw = est[pw];
             const uint32_t hw = sw >> 16;
-            __syncthreads();
             if (lane == 0) hole_f[4 * hw] = kDeadHoleX;
-            // compact the other live electrons (index order kept)
-            int base = 0;
-            for (int r = 0; r < rows; ++r) {
-                const int p = r * kWave + lane;
-                const bool keep = p < n_live && p != pw;
-                const uint32_t st = keep ? est[p] : 0u;
-                const float w = keep ? ew[p] : 0.0f;
-                const uint64_t km = ballot(keep);
-                __syncthreads();
-                if (keep) {
-                    const int q = base + lanes_below(km);
-                    est[q] = st;
-                    ew[q] = w;
-                }
-                base += __popcll(km);
-                __syncthreads();
-            }
-            n_live -= 1;
-            recomb += 1;
-            // electrons that had the removed hole search again (engine.py:168-175)
+            // One pass over the rows: slots pw + 1 .. n_live - 1 move down by one (the
+            // index order kept; a row reads its sources before it writes, and the next
+            // row's sources lie beyond what it wrote), and the electrons whose nearest
+            // hole was hw are queued, at their new slots, for a new search
+            // (engine.py:168-175).
+            const int n_new = n_live - 1;
             int nq = 0;
-            for (int r = 0; r * kWave < n_live; ++r) {
+            for (int r = 0; r * kWave < n_new; ++r) {
                 const int p = r * kWave + lane;
-                const uint32_t st = p < n_live ? est[p] : kDeadSlot;
+                const bool in = p < n_new, moves = in && p >= pw;
+                const uint32_t st = in ? est[moves ? p + 1 : p] : kDeadSlot;
+                const float w = moves ? ew[p + 1] : 0.0f;
+                if (moves) {
+                    est[p] = st;
+                    ew[p] = w;
+                }
                 const bool s = (st >> 16) == hw;
                 const uint64_t m = ballot(s);
                 if (s) rq[nq + lanes_below(m)] = ((uint32_t)p << 16) | (st & 0xFFFFu);
                 nq += __popcll(m);
             }
-            __syncthreads();
+            n_live = n_new;
+            recomb += 1;
+            wave_lds_fence();
             if (nq > 0) {
                 search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane,
                                 dist_evals);
-                __syncthreads();
+                wave_lds_fence();
             }
             ev = 1;
         }
