@@ -78,6 +78,41 @@ int mcl_mlp_train_step(void* mlp, const float* x, const float* t, const int32_t*
 /* Device time of the last forward / grad / train call (HIP events). */
 int mcl_mlp_last_ms(void* mlp, double* ms);
 
+/*
+ * Network-scored differential evolution: the search of the surrogate-assisted
+ * fit.  Replaces the per-candidate objective calls of the reference's scipy DE
+ * (optimizer.py:98-111, objective optimizer.py:66-84) while the network scores
+ * the candidates: the population lives in device memory and a generation is one
+ * propose kernel (DE/rand-to-best/1/bin: mutant = x_a + F (x_best - x_a) +
+ * F (x_b - x_c) over three distinct other members, binomial crossover with one
+ * forced coordinate, coordinates outside [-1, 1] redrawn uniformly;
+ * Philox4x32-10 draws keyed by seed), the network forward, and one
+ * score-and-accept kernel: objective = mean over the n_out outputs of
+ * (expm1(y log_span) out_scale - target)^2 * inv_peak2; a trial replaces its
+ * member when not worse; the argmin goes to the lowest member on ties.
+ * The searched coordinates are the first dim network inputs, in [-1, 1]; the
+ * other n_in - dim inputs are the fixed values.
+ */
+typedef struct mcl_de_params {
+    int32_t pop;              /* members (at least 4; rounded up to 64, at most max_batch) */
+    int32_t dim;              /* searched inputs (1 .. 8, at most n_in)                    */
+    float F, CR;              /* mutation scale [0, 2], crossover probability [0, 1]      */
+    uint64_t seed;            /* Philox key                                                */
+    float fixed[8];           /* inputs dim .. n_in-1 (n_in - dim <= 8)                    */
+    float log_span, out_scale, inv_peak2;
+} mcl_de_params;
+
+/* Creates the search and scores its initial population (uniform in [-1, 1]);
+ * target: DEVICE [n_out], read by every generation. */
+int  mcl_de_create(void* mlp, const mcl_de_params* params, const float* target, void** de_out);
+void mcl_de_destroy(void* de);
+/* Runs gens generations on the context's stream, without a host synchronisation. */
+int  mcl_de_run(void* de, int32_t gens);
+/* The best member (x HOST [dim]) and its objective (f HOST); synchronises. */
+int  mcl_de_best(void* de, float* x, float* f);
+/* The population (x DEVICE [pop][dim]) and objectives (f DEVICE [pop]), copied on the stream. */
+int  mcl_de_population(void* de, float* x, float* f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -115,6 +115,13 @@ class AdamW(C.Structure):
                 ("eps", C.c_double), ("weight_decay", C.c_double), ("step", C.c_int64)]
 
 
+class DeParams(C.Structure):
+    """mcl_de_params (include/mcl_surrogate.h)."""
+    _fields_ = [("pop", C.c_int32), ("dim", C.c_int32), ("F", C.c_float), ("CR", C.c_float),
+                ("seed", C.c_uint64), ("fixed", C.c_float * 8), ("log_span", C.c_float),
+                ("out_scale", C.c_float), ("inv_peak2", C.c_float)]
+
+
 class MclError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"mcl error {code}: {msg}")
@@ -161,6 +168,11 @@ _SIGNATURES = {
     "mcl_mlp_grad": (C.c_int, [_P, _P, _P, _P, C.c_int32, _P, _P]),
     "mcl_mlp_train_step": (C.c_int, [_P, _P, _P, _P, C.c_int32, C.POINTER(AdamW), _P]),
     "mcl_mlp_last_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mcl_de_create": (C.c_int, [_P, C.POINTER(DeParams), _P, C.POINTER(_P)]),
+    "mcl_de_destroy": (None, [_P]),
+    "mcl_de_run": (C.c_int, [_P, C.c_int32]),
+    "mcl_de_best": (C.c_int, [_P, _P, _P]),
+    "mcl_de_population": (C.c_int, [_P, _P, _P]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

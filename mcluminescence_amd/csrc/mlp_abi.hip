@@ -145,6 +145,57 @@ int end(Mlp* m)
     return MCL_OK;
 }
 
+// The forward pass of batch B (a multiple of 64) on the context's stream.
+int forward(Mlp* m, const float* x, int32_t B, float* y)
+{
+    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, m->ctx->stream));
+    const bf16* in = m->X0;
+    const int nl = (int)m->L.size();
+    for (int l = 0; l < nl; ++l) {
+        const Layer& s = m->L[l];
+        M::GemmArgs g{};
+        g.A = in; g.lda = s.kp;
+        g.B = s.wb; g.ldb = s.kp;
+        g.M = B; g.N = s.out; g.K = s.kp;
+        g.bias = s.bw;
+        if (l + 1 < nl) {
+            g.out_a = m->A[l % 2];
+            TRY(gemm(m, g, M::kHiddenInfer));
+            in = m->A[l % 2];
+        } else {
+            g.out_f = y;
+            TRY(gemm(m, g, M::kOutInfer));
+        }
+    }
+    return MCL_OK;
+}
+
+// Network-scored differential evolution: the population and the network's
+// input / output rows of one generation, in device memory.
+struct De {
+    Mlp* m = nullptr;
+    M::DeArgs a{};
+    uint32_t gen = 0;                     // generations run (0: the initial population)
+    std::vector<void*> allocs;
+
+    ~De()
+    {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+};
+
+// One generation on the context's stream: trials, their network curves, selection.
+int de_generation(De* d)
+{
+    Mlp* m = d->m;
+    hipStream_t st = m->ctx->stream;
+    ++d->gen;
+    HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));
+    TRY(forward(m, d->a.X, d->a.pop_pad, const_cast<float*>(d->a.Y)));
+    HIP_TRY(m->ctx, M::de_score(d->a, d->gen, st));
+    return MCL_OK;
+}
+
 // Forward + backward of the batch: gradient partial sums in every layer's `part`
 // (splits recorded in *splits), bias gradients as row sums of dZ^T, loss in *loss.
 int forward_backward(Mlp* m, const float* x, const float* t, const int32_t* idx, int32_t B,
@@ -387,25 +438,7 @@ int mcl_mlp_forward(void* mlp, const float* x, int32_t B, float* y)
     if (!m || !x || !y) return MCL_EINVAL;
     TRY(check_batch(m, B));
     begin(m);
-    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, m->ctx->stream));
-    const bf16* in = m->X0;
-    const int nl = (int)m->L.size();
-    for (int l = 0; l < nl; ++l) {
-        const Layer& s = m->L[l];
-        M::GemmArgs g{};
-        g.A = in; g.lda = s.kp;
-        g.B = s.wb; g.ldb = s.kp;
-        g.M = B; g.N = s.out; g.K = s.kp;
-        g.bias = s.bw;
-        if (l + 1 < nl) {
-            g.out_a = m->A[l % 2];
-            TRY(gemm(m, g, M::kHiddenInfer));
-            in = m->A[l % 2];
-        } else {
-            g.out_f = y;
-            TRY(gemm(m, g, M::kOutInfer));
-        }
-    }
+    TRY(forward(m, x, B, y));
     return end(m);
 }
 
@@ -447,6 +480,121 @@ int mcl_mlp_last_ms(void* mlp, double* ms)
     float f = 0.0f;
     HIP_TRY(m->ctx, hipEventElapsedTime(&f, m->ev0, m->ev1));
     *ms = f;
+    return MCL_OK;
+}
+
+int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void** de_out)
+{
+    Mlp* m = static_cast<Mlp*>(mlp);
+    if (!m || !p || !target || !de_out) return MCL_EINVAL;
+    *de_out = nullptr;
+    const int n_in = m->d.n_in, n_out = m->d.n_out;
+    const int pad = ((p->pop + 63) / 64) * 64;
+    if (p->pop < 4 || pad > m->d.max_batch)
+        return fail(m->ctx, MCL_EINVAL, "population %d must be at least 4 and, rounded up to 64, "
+                    "at most the network's max_batch %d", p->pop, m->d.max_batch);
+    if (p->dim < 1 || p->dim > 8 || p->dim > n_in || n_in - p->dim > 8)
+        return fail(m->ctx, MCL_EINVAL, "dim %d: 1..8 searched inputs of %d, at most 8 fixed",
+                    p->dim, n_in);
+    if (!(p->F >= 0.0f && p->F <= 2.0f) || !(p->CR >= 0.0f && p->CR <= 1.0f))
+        return fail(m->ctx, MCL_EINVAL, "F %g must be in [0, 2] and CR %g in [0, 1]", p->F, p->CR);
+    if (!std::isfinite(p->log_span) || !std::isfinite(p->out_scale) || !(p->inv_peak2 >= 0.0f))
+        return fail(m->ctx, MCL_EINVAL, "log_span, out_scale and inv_peak2 must be finite");
+    De* d = new (std::nothrow) De();
+    if (!d) return fail(m->ctx, MCL_ENOMEM, "host allocation failed");
+    d->m = m;
+    auto alloc = [&](void** q, size_t bytes) -> hipError_t {
+        const hipError_t e = hipMalloc(q, bytes);
+        if (e == hipSuccess) d->allocs.push_back(*q);
+        return e;
+    };
+    M::DeArgs& a = d->a;
+    float* Y = nullptr;
+    hipError_t e = alloc((void**)&a.x, sizeof(float) * (size_t)p->pop * p->dim);
+    if (e == hipSuccess) e = alloc((void**)&a.f, sizeof(float) * (size_t)p->pop);
+    if (e == hipSuccess) e = alloc((void**)&a.X, sizeof(float) * (size_t)pad * n_in);
+    if (e == hipSuccess) e = alloc((void**)&Y, sizeof(float) * (size_t)pad * n_out);
+    if (e == hipSuccess) e = alloc((void**)&a.keys, sizeof(unsigned long long) * 2);
+    if (e != hipSuccess) {
+        delete d;
+        return fail(m->ctx, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "DE buffers: %s",
+                    hipGetErrorString(e));
+    }
+    a.Y = Y;
+    a.target = target;
+    a.pop = p->pop;
+    a.pop_pad = pad;
+    a.dim = p->dim;
+    a.n_in = n_in;
+    a.n_out = n_out;
+    a.F = p->F;
+    a.CR = p->CR;
+    a.log_span = p->log_span;
+    a.out_scale = p->out_scale;
+    a.inv_peak2 = p->inv_peak2;
+    a.key0 = (uint32_t)p->seed;
+    a.key1 = (uint32_t)(p->seed >> 32);
+    for (int k = 0; k < 8; ++k) a.fixed[k] = k < n_in - p->dim ? p->fixed[k] : 0.0f;
+    // generation 0: the initial population, uniform in [-1, 1], scored
+    hipStream_t st = m->ctx->stream;
+    int rc = MCL_OK;
+    if (M::de_init(a, st) != hipSuccess) rc = fail(m->ctx, MCL_EDEVICE, "de_init launch");
+    if (rc == MCL_OK) rc = forward(m, a.X, pad, Y);
+    if (rc == MCL_OK && M::de_score(a, 0u, st) != hipSuccess)
+        rc = fail(m->ctx, MCL_EDEVICE, "de_score launch");
+    if (rc != MCL_OK) {
+        (void)hipStreamSynchronize(st);
+        delete d;
+        return rc;
+    }
+    *de_out = d;
+    return MCL_OK;
+}
+
+void mcl_de_destroy(void* de)
+{
+    De* d = static_cast<De*>(de);
+    if (!d) return;
+    (void)hipStreamSynchronize(d->m->ctx->stream);
+    delete d;
+}
+
+int mcl_de_run(void* de, int32_t gens)
+{
+    De* d = static_cast<De*>(de);
+    if (!d) return MCL_EINVAL;
+    if (gens < 0) return fail(d->m->ctx, MCL_EINVAL, "gens %d < 0", gens);
+    for (int g = 0; g < gens; ++g) TRY(de_generation(d));
+    return MCL_OK;
+}
+
+int mcl_de_best(void* de, float* x, float* f)
+{
+    De* d = static_cast<De*>(de);
+    if (!d || !x || !f) return MCL_EINVAL;
+    Ctx* c = d->m->ctx;
+    unsigned long long key = 0;
+    HIP_TRY(c, hipMemcpyAsync(&key, d->a.keys + (d->gen & 1u), sizeof key, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int i = (int)(uint32_t)key;
+    if (i < 0 || i >= d->a.pop) return fail(c, MCL_EDEVICE, "no best member recorded");
+    HIP_TRY(c, hipMemcpyAsync(x, d->a.x + (size_t)i * d->a.dim, sizeof(float) * d->a.dim,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(f, d->a.f + i, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MCL_OK;
+}
+
+int mcl_de_population(void* de, float* x, float* f)
+{
+    De* d = static_cast<De*>(de);
+    if (!d || !x || !f) return MCL_EINVAL;
+    Ctx* c = d->m->ctx;
+    HIP_TRY(c, hipMemcpyAsync(x, d->a.x, sizeof(float) * (size_t)d->a.pop * d->a.dim,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(f, d->a.f, sizeof(float) * (size_t)d->a.pop,
+                              hipMemcpyDeviceToDevice, c->stream));
     return MCL_OK;
 }
 

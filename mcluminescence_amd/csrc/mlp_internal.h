@@ -88,5 +88,26 @@ __host__ __device__ inline int opt_tiles(int out, int kp) { return (out / 32) * 
 __host__ __device__ inline int opt_bias_groups(int out) { return out / 4; }
 hipError_t optimizer(const OptArgs& a, int n_groups, hipStream_t stream);
 
+// Network-scored differential evolution (de.hip).  x [pop][dim] the population,
+// f [pop] its objectives, X [pop_pad][n_in] the network input (trials in the first
+// dim columns, the fixed inputs after them), Y [pop_pad][n_out] the network output,
+// keys[2] the argmin (objective bits << 32 | member) of the even / odd generations.
+struct DeArgs {
+    float* x;
+    float* f;
+    float* X;
+    const float* Y;
+    const float* target;              // [n_out]
+    unsigned long long* keys;
+    int pop, pop_pad, dim, n_in, n_out;
+    float F, CR;
+    float log_span, out_scale, inv_peak2;
+    uint32_t key0, key1;
+    float fixed[8];
+};
+hipError_t de_init(const DeArgs& a, hipStream_t stream);
+hipError_t de_propose(const DeArgs& a, uint32_t gen, hipStream_t stream);
+hipError_t de_score(const DeArgs& a, uint32_t gen, hipStream_t stream);
+
 }  // namespace mlp
 }  // namespace mcl

@@ -21,10 +21,11 @@ E, s, rho' and fill.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from dataclasses import dataclass, field
-from typing import Callable, List, Optional
+from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
@@ -109,51 +110,69 @@ class DifferentialEvolution:
         return self.x[i].copy(), float(self.f[i])
 
 
-class DeviceDifferentialEvolution:
-    """The same DE/rand-to-best/1/bin step with the population kept in device
-    memory (torch tensors), so a generation of the network-scored search issues a
-    few asynchronous kernels and no host round trip.  Partners are three distinct
-    offsets from the target, so each target's partners differ from it and from
-    one another."""
+class NetworkDE:
+    """The network-scored search of the surrogate-assisted fit, in device memory
+    (mcl_de_*, include/mcl_surrogate.h): the DE/rand-to-best/1/bin step of
+    DifferentialEvolution, with the population, the trials and the network's
+    curves on the GPU.  A generation is a propose kernel, the network forward and
+    a score-and-accept kernel; run(k) queues k generations with no host round
+    trip.  The objective is objective() of the network's de-normalised curve:
+    mean over bins of (curve - y)^2 / peak(y)^2."""
 
-    def __init__(self, d: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3):
+    def __init__(self, net: S.Network, y: np.ndarray, scale: float, fixed: Sequence[float],
+                 dim: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3):
         import torch
-        self.torch = torch
-        self.d, self.P, self.F, self.CR = d, pop, F, CR
-        self.g = torch.Generator(device="cuda").manual_seed(int(seed))
-        self.x = torch.rand((pop, d), generator=self.g, device="cuda", dtype=torch.float32) * 2 - 1
-        self.f = torch.full((pop,), float("inf"), device="cuda", dtype=torch.float64)
-        self.idx = torch.arange(pop, device="cuda")
+        from ._abi import DeParams
+        self.net, self.dim, self.pop = net, dim, pop
+        self.target = torch.from_numpy(np.asarray(y, np.float32)).cuda()
+        p = DeParams()
+        p.pop, p.dim, p.F, p.CR, p.seed = pop, dim, F, CR, int(seed) & (2 ** 64 - 1)
+        for k, v in enumerate(fixed):
+            p.fixed[k] = float(v)
+        p.log_span = math.log1p(1.0 / net.c0)
+        p.out_scale = net.c0 * scale
+        p.inv_peak2 = 1.0 / max(float(np.max(y)), 1.0) ** 2
+        h = C.c_void_p()
+        with net._on_stream(self.target):
+            net.ctx.check(net.lib.mcl_de_create(net.handle, C.byref(p), self.target.data_ptr(),
+                                                C.byref(h)), "mcl_de_create")
+        self.handle = h
 
-    def _randint(self, lo: int, hi: int):
-        return self.torch.randint(lo, hi, (self.P,), generator=self.g, device="cuda")
-
-    def propose(self):
-        t, P, d = self.torch, self.P, self.d
-        best = self.x[t.argmin(self.f)]
-        a = self._randint(1, P)
-        b = self._randint(1, P - 1)
-        b = b + (b >= a).long()
-        c = self._randint(1, P - 2)
-        lo, hi = t.minimum(a, b), t.maximum(a, b)
-        c = c + (c >= lo).long()
-        c = c + (c >= hi).long()
-        x0, x1, x2 = self.x[(self.idx + a) % P], self.x[(self.idx + b) % P], self.x[(self.idx + c) % P]
-        mutant = x0 + self.F * (best - x0) + self.F * (x1 - x2)
-        cross = t.rand((P, d), generator=self.g, device="cuda") < self.CR
-        cross[self.idx, t.randint(0, d, (P,), generator=self.g, device="cuda")] = True
-        trial = t.where(cross, mutant, self.x)
-        redraw = t.rand((P, d), generator=self.g, device="cuda", dtype=t.float32) * 2 - 1
-        return t.where((trial < -1) | (trial > 1), redraw, trial)
-
-    def accept(self, trial, f) -> None:
-        better = f <= self.f
-        self.x = self.torch.where(better[:, None], trial, self.x)
-        self.f = self.torch.where(better, f, self.f)
+    def run(self, gens: int) -> None:
+        with self.net._on_stream(self.target):
+            self.net.ctx.check(self.net.lib.mcl_de_run(self.handle, int(gens)), "mcl_de_run")
 
     def best(self):
-        i = int(self.torch.argmin(self.f).item())
-        return self.x[i].double().cpu().numpy(), float(self.f[i].item())
+        """(coordinates [dim] float64, objective) of the best member."""
+        x = np.zeros(self.dim, np.float32)
+        f = C.c_float()
+        with self.net._on_stream(self.target):
+            self.net.ctx.check(self.net.lib.mcl_de_best(self.handle, x.ctypes.data, C.byref(f)),
+                               "mcl_de_best")
+        return x.astype(np.float64), float(f.value)
+
+    def population(self):
+        """(x [pop][dim], f [pop]) as CUDA float32 tensors."""
+        import torch
+        x = torch.empty((self.pop, self.dim), dtype=torch.float32, device="cuda")
+        f = torch.empty(self.pop, dtype=torch.float32, device="cuda")
+        with self.net._on_stream(self.target, x, f):
+            self.net.ctx.check(self.net.lib.mcl_de_population(self.handle, x.data_ptr(),
+                                                              f.data_ptr()), "mcl_de_population")
+        return x, f
+
+    def close(self) -> None:
+        """Free the search (after the work queued on the network's stream)."""
+        if getattr(self, "handle", None):
+            with self.net._on_stream():
+                self.net.lib.mcl_de_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -218,33 +237,24 @@ def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, bet
     candidate, Monte Carlo fallback when the network misses the judge by more than
     gate (NRMSE).  Every `patience` generations the best objective is read back;
     the search stops once it has improved by less than 1e-4 of itself since."""
-    import torch
     t0 = time.time()
-    de = DeviceDifferentialEvolution(N_FIT, pop, seed)
     # the encoded fit coordinates are the network's inputs; beta is the sixth
     u_beta = float(S.encode(full_theta(np.zeros(N_FIT), beta))[0, 5])
-    y_dev = torch.from_numpy(np.asarray(y, np.float32)).cuda()
-    inv_peak2 = 1.0 / max(float(y.max()), 1.0) ** 2
-    log_span = math.log1p(1.0 / net.c0)
-    beta_col = torch.full((pop, 1), u_beta, device="cuda")
-
-    def net_eval(u5):
-        yq = net.forward(torch.cat([u5, beta_col], dim=1))
-        d = torch.expm1(yq * log_span) * (net.c0 * scale) - y_dev     # objective() on device
-        return ((d * d).mean(dim=1) * inv_peak2).double()
-
-    de.accept(de.x, net_eval(de.x))
-    hist = [float(de.f.min().item())]
-    g = 0
-    while g < gens:
-        for _ in range(min(patience, gens - g)):
-            trial = de.propose()
-            de.accept(trial, net_eval(trial))
-            g += 1
-        hist.append(float(de.f.min().item()))
-        if hist[-1] > hist[-2] * (1.0 - 1e-4):
-            break
-    u, _ = de.best()
+    de = NetworkDE(net, y, scale, [u_beta], N_FIT, pop, seed)
+    try:
+        hist = [de.best()[1]]
+        g = 0
+        while g < gens:
+            k = min(patience, gens - g)
+            de.run(k)
+            g += k
+            hist.append(de.best()[1])
+            if hist[-1] > hist[-2] * (1.0 - 1e-4):
+                break
+        u, _ = de.best()
+        xs, fs = de.population()        # the fallback's starting members
+    finally:
+        de.close()
     th = full_theta(u, beta)
     judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
     pred = S.predict_curves(net, th)[0]
@@ -256,8 +266,9 @@ def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, bet
         return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
                          1, n_net, False, hist)
     # fallback: Monte Carlo generations from the network's best candidates
-    order = torch.argsort(de.f)[:fallback_pop]
+    import torch
+    order = torch.argsort(fs)[:fallback_pop]
     res = fit_mc(ctx, y, scale, beta, pop=fallback_pop, gens=fallback_gens, seed=seed + 1,
-                 init=de.x[order].double().cpu().numpy(), log=log)
+                 init=xs[order].double().cpu().numpy(), log=log)
     return FitResult(res.theta, res.objective_mc, err, time.time() - t0,
                      1 + res.mc_candidates, n_net, True, hist + res.history)
