@@ -30,10 +30,6 @@ __device__ __forceinline__ uint32_t below(uint32_t w, uint32_t n)
 __global__ __launch_bounds__(256) void de_init_kernel(DeArgs a)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i == 0) {
-        a.keys[0] = ~0ull;
-        a.keys[1] = ~0ull;
-    }
     if (i >= a.pop_pad) return;
     float* X = a.X + (int64_t)i * a.n_in;
     if (i >= a.pop) {
@@ -52,13 +48,33 @@ __global__ __launch_bounds__(256) void de_init_kernel(DeArgs a)
     a.f[i] = __builtin_inff();
 }
 
+// Population argmin from the members' keys (objective bits << 32 | member; the
+// objectives are >= 0, so their float bits order as the values, and ties go to
+// the lowest member), reduced by each workgroup: a strided minimum per thread,
+// then across the wave by shuffles and across the waves in LDS.
+__device__ __forceinline__ int population_argmin(const DeArgs& a)
+{
+    __shared__ unsigned long long wmin[256 / 64];
+    unsigned long long k = ~0ull;
+    for (int j = (int)threadIdx.x; j < a.pop; j += (int)blockDim.x) k = a.keys[j] < k ? a.keys[j] : k;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long v = (unsigned long long)__shfl_xor((long long)k, o, 64);
+        k = v < k ? v : k;
+    }
+    if ((threadIdx.x & 63u) == 0u) wmin[threadIdx.x >> 6] = k;
+    __syncthreads();
+    k = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) k = wmin[w] < k ? wmin[w] : k;
+    return (int)(uint32_t)k;
+}
+
 __global__ __launch_bounds__(256) void de_propose_kernel(DeArgs a, uint32_t gen)
 {
+    const int best = population_argmin(a);          // of the previous generation
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i == 0) a.keys[gen & 1u] = ~0ull;          // this generation's argmin starts empty
     if (i >= a.pop) return;
     const uint32_t P = (uint32_t)a.pop;
-    const int best = (int)(uint32_t)a.keys[(gen - 1u) & 1u];   // the previous generation's
     const u32x4 r = philox10(gen, (uint32_t)i, 0u, 0u, a.key0, a.key1);
     // three distinct offsets in [1, P): partners differ from i and from one another
     const uint32_t oa = 1u + below(r.x, P - 1u);
@@ -93,10 +109,8 @@ __global__ __launch_bounds__(256) void de_propose_kernel(DeArgs a, uint32_t gen)
 }
 
 // One wave per member: objective of its trial (the network output row), greedy
-// selection, and the population argmin by a 64-bit atomic minimum of
-// (objective bits, member): objectives are >= 0, so their float bits order as
-// the values, and ties go to the lowest member.
-__global__ __launch_bounds__(256) void de_score_kernel(DeArgs a, uint32_t gen)
+// selection, and the member's argmin key.
+__global__ __launch_bounds__(256) void de_score_kernel(DeArgs a)
 {
     const int lane = (int)(threadIdx.x & 63u);
     const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -116,8 +130,7 @@ __global__ __launch_bounds__(256) void de_score_kernel(DeArgs a, uint32_t gen)
     if (lane == 0) {
         const float fn = take ? ft : fo;
         if (take) a.f[i] = ft;
-        atomicMin(&a.keys[gen & 1u],
-                  ((unsigned long long)__float_as_uint(fn) << 32) | (unsigned long long)i);
+        a.keys[i] = ((unsigned long long)__float_as_uint(fn) << 32) | (unsigned long long)i;
     }
 }
 
@@ -137,10 +150,10 @@ hipError_t de_propose(const DeArgs& a, uint32_t gen, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t de_score(const DeArgs& a, uint32_t gen, hipStream_t stream)
+hipError_t de_score(const DeArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL(de_score_kernel, dim3((unsigned)((a.pop + 3) / 4)), dim3(256), 0, stream,
-                       a, gen);
+                       a);
     return hipGetLastError();
 }
 

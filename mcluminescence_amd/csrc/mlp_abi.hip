@@ -192,7 +192,7 @@ int de_generation(De* d)
     ++d->gen;
     HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));
     TRY(forward(m, d->a.X, d->a.pop_pad, const_cast<float*>(d->a.Y)));
-    HIP_TRY(m->ctx, M::de_score(d->a, d->gen, st));
+    HIP_TRY(m->ctx, M::de_score(d->a, st));
     return MCL_OK;
 }
 
@@ -514,7 +514,7 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
     if (e == hipSuccess) e = alloc((void**)&a.f, sizeof(float) * (size_t)p->pop);
     if (e == hipSuccess) e = alloc((void**)&a.X, sizeof(float) * (size_t)pad * n_in);
     if (e == hipSuccess) e = alloc((void**)&Y, sizeof(float) * (size_t)pad * n_out);
-    if (e == hipSuccess) e = alloc((void**)&a.keys, sizeof(unsigned long long) * 2);
+    if (e == hipSuccess) e = alloc((void**)&a.keys, sizeof(unsigned long long) * (size_t)p->pop);
     if (e != hipSuccess) {
         delete d;
         return fail(m->ctx, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "DE buffers: %s",
@@ -540,7 +540,7 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
     int rc = MCL_OK;
     if (M::de_init(a, st) != hipSuccess) rc = fail(m->ctx, MCL_EDEVICE, "de_init launch");
     if (rc == MCL_OK) rc = forward(m, a.X, pad, Y);
-    if (rc == MCL_OK && M::de_score(a, 0u, st) != hipSuccess)
+    if (rc == MCL_OK && M::de_score(a, st) != hipSuccess)
         rc = fail(m->ctx, MCL_EDEVICE, "de_score launch");
     if (rc != MCL_OK) {
         (void)hipStreamSynchronize(st);
@@ -573,12 +573,14 @@ int mcl_de_best(void* de, float* x, float* f)
     De* d = static_cast<De*>(de);
     if (!d || !x || !f) return MCL_EINVAL;
     Ctx* c = d->m->ctx;
-    unsigned long long key = 0;
-    HIP_TRY(c, hipMemcpyAsync(&key, d->a.keys + (d->gen & 1u), sizeof key, hipMemcpyDeviceToHost,
+    std::vector<float> fh((size_t)d->a.pop);
+    HIP_TRY(c, hipMemcpyAsync(fh.data(), d->a.f, sizeof(float) * fh.size(), hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    const int i = (int)(uint32_t)key;
-    if (i < 0 || i >= d->a.pop) return fail(c, MCL_EDEVICE, "no best member recorded");
+    // first index of the minimum, as the kernels' argmin (unordered objectives last)
+    int i = 0;
+    for (int j = 1; j < d->a.pop; ++j)
+        if (fh[(size_t)j] < fh[(size_t)i] || (std::isnan(fh[(size_t)i]) && !std::isnan(fh[(size_t)j]))) i = j;
     HIP_TRY(c, hipMemcpyAsync(x, d->a.x + (size_t)i * d->a.dim, sizeof(float) * d->a.dim,
                               hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(f, d->a.f + i, sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -91,7 +91,7 @@ hipError_t optimizer(const OptArgs& a, int n_groups, hipStream_t stream);
 // Network-scored differential evolution (de.hip).  x [pop][dim] the population,
 // f [pop] its objectives, X [pop_pad][n_in] the network input (trials in the first
 // dim columns, the fixed inputs after them), Y [pop_pad][n_out] the network output,
-// keys[2] the argmin (objective bits << 32 | member) of the even / odd generations.
+// keys [pop] each member's (objective bits << 32 | member) after the last scoring.
 struct DeArgs {
     float* x;
     float* f;
@@ -107,7 +107,7 @@ struct DeArgs {
 };
 hipError_t de_init(const DeArgs& a, hipStream_t stream);
 hipError_t de_propose(const DeArgs& a, uint32_t gen, hipStream_t stream);
-hipError_t de_score(const DeArgs& a, uint32_t gen, hipStream_t stream);
+hipError_t de_score(const DeArgs& a, hipStream_t stream);
 
 }  // namespace mlp
 }  // namespace mcl
