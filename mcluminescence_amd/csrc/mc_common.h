@@ -188,71 +188,63 @@ __device__ __forceinline__ uint64_t block_search_wave(const Grid& g, float x, fl
     return key;
 }
 
-// Wave-cooperative 3x3x3 block searches for up to kSearchBatch electrons at once
-// (electron b's coordinates in lane b).  A block is at most 9 contiguous slot
-// ranges (one per (z, y) row of cells); lane l takes range l / 7 at offsets l % 7,
-// l % 7 + 7, ... (9 x 7 = 63 lanes), finding its slots from two cell starts.  The
-// electrons' loads, first two passes and wave minima are independent, so their
-// LDS round trips overlap; longer ranges take further passes one electron at a
-// time.  Returns electron b's block minimum key in keys[b] (wave-uniform).
+// Wave-cooperative 3x3x3 block searches for up to kSearchBatch (four) electrons at
+// once, one 16-lane row each (x, y, z: the row's electron, the same in all its
+// lanes).  A block is at most 9 contiguous slot ranges (one per (z, y) row of
+// cells); lane q of the row takes offsets q, q + 16, ... of every range, so its 9
+// range loads are independent; ranges longer than 16 holes take further passes.
+// The row minimum is four DPP steps.  Returns the row's block minimum key in every
+// lane of the row (~0 for rows at or beyond nb).
 constexpr int kSearchBatch = 4;
-__device__ __forceinline__ void block_search_batch(const Grid& g, int nb, float qx, float qy,
-                                                   float qz, int lane,
-                                                   uint64_t (&keys)[kSearchBatch],
-                                                   uint64_t& evals)
+__device__ __forceinline__ uint64_t block_search_rows(const Grid& g, int nb, float x, float y,
+                                                      float z, int lane, uint64_t& evals)
 {
-    constexpr int kPer = 7;
     const int G = g.G;
-    const int j = lane / kPer, o = lane - j * kPer;
-    int s[kSearchBatch], e[kSearchBatch];
-    float bx[kSearchBatch], by[kSearchBatch], bz[kSearchBatch];
+    const int q = lane & 15;
+    const bool act = (lane >> 4) < nb;
+    int s[9], e[9];
+    int longest = 0;
+    if (act) {
+        const Block k(g, x, y, z);
+        const int cz = cell_of(z, g.inv_cell, G), cy = cell_of(y, g.inv_cell, G);
+        int total = 0;
 #pragma unroll
-    for (int b = 0; b < kSearchBatch; ++b) {
-        bx[b] = lane_value(qx, b);
-        by[b] = lane_value(qy, b);
-        bz[b] = lane_value(qz, b);
-        s[b] = 0;
-        e[b] = 0;
-        if (b < nb) {
-            const Block k(g, bx[b], by[b], bz[b]);
-            // (z, y) row j of the unclamped block; rows outside the grid are empty
-            const int cz = cell_of(bz[b], g.inv_cell, G), cy = cell_of(by[b], g.inv_cell, G);
-            const int jz = j / 3, zz = cz - 1 + jz, yy = cy - 1 + (j - 3 * jz);
-            if (j < 9 && (unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
+        for (int r = 0; r < 9; ++r) {
+            // (z, y) row r of the unclamped block; rows outside the grid are empty
+            const int zz = cz - 1 + r / 3, yy = cy - 1 + r % 3;
+            s[r] = 0;
+            e[r] = 0;
+            if ((unsigned)zz < (unsigned)G && (unsigned)yy < (unsigned)G) {
                 const int base = (zz * G + yy) * G;
-                s[b] = (int)g.cstart[base + k.x0];
-                e[b] = (int)g.cstart[base + k.x1 + 1];
+                s[r] = (int)g.cstart[base + k.x0];
+                e[r] = (int)g.cstart[base + k.x1 + 1];
             }
-            if (o == 0) evals += (uint64_t)(e[b] - s[b]);   // summed over lanes at the end
+            total += e[r] - s[r];
+            longest = max(longest, e[r] - s[r]);
+        }
+        if (q == 0) evals += (uint64_t)total;      // summed over lanes at the end
+    } else {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) s[r] = e[r] = 0;
+    }
+    uint64_t key = ~0ull;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int slot = s[r] + q;
+        if (slot < e[r]) {
+            const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
+            key = k < key ? k : key;
         }
     }
-    uint64_t key[kSearchBatch];
-#pragma unroll
-    for (int b = 0; b < kSearchBatch; ++b) {
-        key[b] = ~0ull;
-#pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int slot = s[b] + o + pass * kPer;
-            if (slot < e[b]) {
-                const uint64_t k = key_of(g.hole[slot], bx[b], by[b], bz[b], slot);
-                key[b] = k < key[b] ? k : key[b];
-            }
-        }
-    }
-#pragma unroll
-    for (int b = 0; b < kSearchBatch; ++b) {
-        if (b >= nb) break;
-        for (int slot = s[b] + o + 2 * kPer;; slot += kPer) {
-            const bool v = slot < e[b];
-            if (ballot(v) == 0) break;
-            if (v) {
-                const uint64_t k = key_of(g.hole[slot], bx[b], by[b], bz[b], slot);
-                key[b] = k < key[b] ? k : key[b];
+    if (ballot(longest > 16) != 0) {
+        for (int r = 0; r < 9; ++r) {
+            for (int slot = s[r] + q + 16; slot < e[r]; slot += 16) {
+                const uint64_t k = key_of(g.hole[slot], x, y, z, slot);
+                key = k < key ? k : key;
             }
         }
     }
-#pragma unroll
-    for (int b = 0; b < kSearchBatch; ++b) keys[b] = b < nb ? wave_min_u64(key[b]) : ~0ull;
+    return row_min_u64(key);
 }
 
 // The whole wave scans every hole for one electron (x, y, z wave-uniform).
@@ -474,39 +466,46 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
     if (nq <= wave_search_max) {
         for (int i0 = 0; i0 < nq; i0 += kSearchBatch) {
             const int nb = min(kSearchBatch, nq - i0);
-            const bool mine = lane < nb;               // lane b carries electron b
-            const uint32_t qq = mine ? rq[i0 + lane] : 0u;
-            float qx = 0.0f, qy = 0.0f, qz = 0.0f;
-            if (mine) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
-            uint64_t key;
-            uint64_t fb;
             if (nb == 1) {
-                // one electron: the single search ends after the passes its ranges need
+                // one electron: the whole wave takes its block (9 ranges x 7 lanes)
+                const uint32_t qq = rq[i0];
+                float qx, qy, qz;
+                egen.xyz(qq & 0xFFFFu, qx, qy, qz);
                 bool ok = true;
-                key = block_search_wave(grid, lane_value(qx, 0), lane_value(qy, 0),
-                                        lane_value(qz, 0), lane, ok, dist_evals);
-                fb = ok ? 0ull : 1ull;
-            } else {
-                uint64_t keys[kSearchBatch];
-                block_search_batch(grid, nb, qx, qy, qz, lane, keys, dist_evals);
-                key = keys[0];
-#pragma unroll
-                for (int b = 1; b < kSearchBatch; ++b) key = lane == b ? keys[b] : key;
-                bool ok = true;
-                if (mine) {
-                    const Block k(grid, qx, qy, qz);
-                    ok = k.conclusive(grid, qx, qy, qz, __uint_as_float((uint32_t)(key >> 32)));
+                uint64_t key = block_search_wave(grid, qx, qy, qz, lane, ok, dist_evals);
+                if (!ok) key = full_scan_wave(grid, qx, qy, qz, lane, dist_evals);
+                if (lane == 0) {
+                    uint32_t slot;
+                    float w;
+                    nn_from_key(key, alpha_f, slot, w);
+                    est[qq >> 16] = (slot << 16) | (qq & 0xFFFFu);
+                    ew[qq >> 16] = w;
                 }
-                fb = ballot(mine && !ok);
+                continue;
             }
+            // two to four electrons: one 16-lane row each; every lane of row b
+            // regenerates electron b's coordinates (the same Philox call in all lanes)
+            const int b = lane >> 4;
+            const bool act = b < nb;
+            const uint32_t qq = act ? rq[i0 + b] : 0u;
+            float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+            if (act) egen.xyz(qq & 0xFFFFu, qx, qy, qz);
+            uint64_t key = block_search_rows(grid, nb, qx, qy, qz, lane, dist_evals);
+            bool ok = true;
+            if (act) {
+                const Block k(grid, qx, qy, qz);
+                ok = k.conclusive(grid, qx, qy, qz, __uint_as_float((uint32_t)(key >> 32)));
+            }
+            // rows whose block was inconclusive: the whole wave scans every hole
+            uint64_t fb = ballot((lane & 15) == 0 && act && !ok);
             while (fb != 0) {
                 const int l = __builtin_ctzll(fb);
                 fb &= fb - 1;
                 const uint64_t k = full_scan_wave(grid, lane_value(qx, l), lane_value(qy, l),
                                                   lane_value(qz, l), lane, dist_evals);
-                if (lane == l) key = k;
+                if ((lane >> 4) == (l >> 4)) key = k;
             }
-            if (mine) {
+            if (act && (lane & 15) == 0) {
                 uint32_t slot;
                 float w;
                 nn_from_key(key, alpha_f, slot, w);

@@ -260,6 +260,22 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
     return ((uint64_t)mh << 32) | ml;
 }
 
+// Minimum of a 64-bit key over each 16-lane row (all lanes active), in every lane
+// of the row: four DPP steps (pairs, quads, half rows, rows).
+__device__ __forceinline__ uint64_t row_min_u64(uint64_t v)
+{
+    uint64_t u = ((uint64_t)dpp_u32<kDppQuad1032>((uint32_t)(v >> 32)) << 32) |
+                 dpp_u32<kDppQuad1032>((uint32_t)v);
+    v = u < v ? u : v;
+    u = ((uint64_t)dpp_u32<kDppQuad2301>((uint32_t)(v >> 32)) << 32) | dpp_u32<kDppQuad2301>((uint32_t)v);
+    v = u < v ? u : v;
+    u = ((uint64_t)dpp_u32<kDppRowHalfMirror>((uint32_t)(v >> 32)) << 32) |
+        dpp_u32<kDppRowHalfMirror>((uint32_t)v);
+    v = u < v ? u : v;
+    u = ((uint64_t)dpp_u32<kDppRowMirror>((uint32_t)(v >> 32)) << 32) | dpp_u32<kDppRowMirror>((uint32_t)v);
+    return u < v ? u : v;
+}
+
 __device__ __forceinline__ uint64_t wave_or_u64(uint64_t v)
 {
 #pragma unroll
