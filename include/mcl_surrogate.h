@@ -91,7 +91,9 @@ int mcl_mlp_last_ms(void* mlp, double* ms);
  * (expm1(y log_span) out_scale - target)^2 * inv_peak2; a trial replaces its
  * member when not worse; the argmin goes to the lowest member on ties.
  * The searched coordinates are the first dim network inputs, in [-1, 1]; the
- * other n_in - dim inputs are the fixed values.
+ * other n_in - dim inputs are the fixed values.  A search runs the network on
+ * the network's own activation buffers: calls of the same network (another
+ * search, forward, grad, train) must not overlap it on another stream.
  */
 typedef struct mcl_de_params {
     int32_t pop;              /* members (at least 4; rounded up to 64, at most max_batch) */
