@@ -208,7 +208,27 @@ def library() -> C.CDLL:
     return lib
 
 
-def run_array(runs: Sequence[RunParams]):
+# numpy view of mcl_run_params: a structured table of runs, built column-wise, is
+# handed to the library as a ctypes array without a per-run Python object
+RUN_DTYPE = np.dtype([(name, np.float64 if ct is C.c_double else np.int32)
+                      for name, ct in RunParams._fields_])
+assert RUN_DTYPE.itemsize == C.sizeof(RunParams)
+
+
+def run_table(n: int) -> np.ndarray:
+    """A zeroed table of n runs (structured array, RUN_DTYPE)."""
+    return np.zeros(n, dtype=RUN_DTYPE)
+
+
+def run_array(runs):
+    """ctypes array of mcl_run_params from a list of RunParams, a RUN_DTYPE table
+    (copied) or a ctypes array (passed through)."""
+    if isinstance(runs, C.Array):
+        return runs
+    if isinstance(runs, np.ndarray):
+        if runs.dtype != RUN_DTYPE:
+            raise TypeError("run table must have dtype RUN_DTYPE")
+        return (RunParams * len(runs)).from_buffer_copy(np.ascontiguousarray(runs))
     arr = (RunParams * len(runs))()
     for i, r in enumerate(runs):
         arr[i] = r
@@ -279,8 +299,12 @@ class Context:
         arr = run_array(runs)
         if n_bins is None:
             n_bins = 0
-        total = sum(int(r.sims) for r in runs)
-        row_len = n_bins if n_bins > 0 else max(int(r.n_steps) for r in runs if r.sims > 0)
+        if isinstance(runs, np.ndarray):
+            total = int(runs["sims"].sum())
+            row_len = n_bins if n_bins > 0 else int(runs["n_steps"][runs["sims"] > 0].max())
+        else:
+            total = sum(int(r.sims) for r in runs)
+            row_len = n_bins if n_bins > 0 else max(int(r.n_steps) for r in runs if r.sims > 0)
         counts = np.zeros((total, row_len), dtype=np.uint32)
         live = np.zeros(total, dtype=np.uint64)
         stats = Stats()

@@ -28,7 +28,7 @@ from typing import Callable, Optional, Sequence, Tuple
 import numpy as np
 
 from . import workloads as W
-from ._abi import DEFAULT_SEED, F_PERSISTENT, AdamW, Context, MclError, MlpDesc, RunParams
+from ._abi import DEFAULT_SEED, F_PERSISTENT, AdamW, Context, MclError, MlpDesc, RunParams, run_table
 
 N_IN = 6
 N_BINS = 256
@@ -103,6 +103,23 @@ def tl_run(theta: Sequence[float], n: int, sims: int) -> RunParams:
                          n_steps=N_STEPS, sims=sims, fill=fill)
 
 
+def tl_table(theta: np.ndarray, n: int, sims: int) -> np.ndarray:
+    """tl_run for every row of theta [N, 6], as a run table (RUN_DTYPE), built
+    column-wise; the same numbers as the per-set runs (a test compares the bytes)."""
+    theta = np.atleast_2d(np.asarray(theta, np.float64))
+    E, s, alpha, rho, fill, beta = (theta[:, j] for j in range(6))
+    t = run_table(theta.shape[0])
+    t["alpha"], t["b"], t["s"] = alpha, s, s
+    t["E_cb"], t["E_loc_1"], t["E_loc_2"] = E, E, E
+    t["D0"], t["Retrap"], t["k_b"], t["D"] = W.D0, 0.5, W.K_B, 0.0
+    # the scalar formula per set (its pow and division, as single_trap computes them)
+    t["rho_prime"] = [W.rho_prime_for(float(r), float(a)) for r, a in zip(rho, alpha)]
+    t["holes"], t["boundary_factor"], t["N_e"], t["e_ratio_start"] = float(n), 1.2, float(n), fill
+    t["T_start"], t["T_rate"], t["dt"] = T_LO - 273.15, beta, (T_HI - T_LO) / beta / N_STEPS
+    t["n_steps"], t["sims"] = N_STEPS, sims
+    return t
+
+
 def mc_curves(ctx: Context, theta: np.ndarray, n: int = 400, sims: int = 8,
               seed: int = DEFAULT_SEED, chunk: int = 8192) -> np.ndarray:
     """Monte Carlo glow curves [N, 256]: recombinations per bin summed over the
@@ -113,7 +130,7 @@ def mc_curves(ctx: Context, theta: np.ndarray, n: int = 400, sims: int = 8,
     out = np.empty((theta.shape[0], N_BINS), np.float64)
     for c0 in range(0, theta.shape[0], chunk):
         part = theta[c0:c0 + chunk]
-        runs = [tl_run(t, n, sims) for t in part]
+        runs = tl_table(part, n, sims)
         counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=F_PERSISTENT)
         out[c0:c0 + len(part)] = counts.reshape(len(part), sims, N_BINS).sum(axis=1) / (sims * n)
     return out

@@ -71,3 +71,17 @@ def test_recovery_uses_rho_prime():
     r = F.FitResult(th2, 0.0, 0.0, 0.0, 0, 0).recovery(th)
     assert abs(rho_prime_for(th2[3], th2[2]) / rho_prime_for(th[3], th[2]) - 1) < 1e-12
     assert r["log10_rho_prime"] < 1e-12 and r["E"] == 0
+
+
+def test_run_table_is_byte_identical_to_the_per_set_runs():
+    """surrogate.tl_table (column-wise) gives exactly the mcl_run_params of tl_run
+    per parameter set, and run tables pass through run_array unchanged."""
+    from mcluminescence_amd import surrogate as S
+    from mcluminescence_amd._abi import RUN_DTYPE, run_array
+    theta = S.sample_params(257, np.random.default_rng(5))
+    per_set = bytes(run_array([S.tl_run(th, 400, 8) for th in theta]))
+    table = S.tl_table(theta, 400, 8)
+    assert table.dtype == RUN_DTYPE
+    assert bytes(run_array(table)) == per_set
+    arr = run_array(table)
+    assert run_array(arr) is arr
