@@ -3,6 +3,9 @@
 // propose kernel, the network forward (mlp.hip) and one score-and-accept kernel,
 // with no host round trip.
 //
+// The propose kernel writes the trials straight into the network's packed BF16
+// input rows (the conversion pack_input makes), so a generation needs no pack pass.
+//
 // DE/rand-to-best/1/bin, the step of scipy's differential_evolution that the
 // reference's fit calls (src/class/optimizer.py:98-111):
 //   mutant  = x_a + F (x_best - x_a) + F (x_b - x_c), a, b, c distinct and != i
@@ -95,7 +98,10 @@ __global__ __launch_bounds__(256) void de_propose_kernel(DeArgs a, uint32_t gen)
     const u32x4 d0 = philox10(gen, (uint32_t)i, 3u, 0u, a.key0, a.key1);   // redraws
     const u32x4 d1 = philox10(gen, (uint32_t)i, 4u, 0u, a.key0, a.key1);
     float* X = a.X + (int64_t)i * a.n_in;
-    for (int j = 0; j < a.dim; ++j) {
+    float in[16];                                    // network inputs 0..15 of the row
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j >= a.dim) break;
         const float m = x0[j] + a.F * (xb[j] - x0[j]) + a.F * (x1[j] - x2[j]);
         const uint32_t uc = j < 4 ? pick_word(c0, j) : pick_word(c1, j - 4);
         const bool cross = unit_f(uc) < a.CR || j == jr;
@@ -106,6 +112,24 @@ __global__ __launch_bounds__(256) void de_propose_kernel(DeArgs a, uint32_t gen)
         }
         X[j] = t;
     }
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        in[k] = k < a.dim ? X[k] : (k < a.n_in ? a.fixed[k - a.dim] : 0.0f);
+    // the whole packed row, as pack_input writes it (trials, fixed inputs, zeros),
+    // eight BF16 values per 16-byte store
+    uint4* X0 = reinterpret_cast<uint4*>(a.X0 + (int64_t)i * a.kp0);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        uint32_t w[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const __bf16 lo = (__bf16)in[8 * c + 2 * t], hi = (__bf16)in[8 * c + 2 * t + 1];
+            w[t] = (uint32_t)__builtin_bit_cast(unsigned short, lo) |
+                   ((uint32_t)__builtin_bit_cast(unsigned short, hi) << 16);
+        }
+        X0[c] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    for (int c = 2; c < a.kp0 / 8; ++c) X0[c] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // One wave per member: objective of its trial (the network output row), greedy

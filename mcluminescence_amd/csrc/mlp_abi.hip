@@ -145,10 +145,10 @@ int end(Mlp* m)
     return MCL_OK;
 }
 
-// The forward pass of batch B (a multiple of 64) on the context's stream.
-int forward(Mlp* m, const float* x, int32_t B, float* y)
+// The layers of the forward pass on the packed input m->X0 (batch B, a multiple
+// of 64), on the context's stream.
+int forward_packed(Mlp* m, int32_t B, float* y)
 {
-    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, m->ctx->stream));
     const bf16* in = m->X0;
     const int nl = (int)m->L.size();
     for (int l = 0; l < nl; ++l) {
@@ -168,6 +168,13 @@ int forward(Mlp* m, const float* x, int32_t B, float* y)
         }
     }
     return MCL_OK;
+}
+
+// The forward pass of batch B: pack x into m->X0, then the layers.
+int forward(Mlp* m, const float* x, int32_t B, float* y)
+{
+    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, m->ctx->stream));
+    return forward_packed(m, B, y);
 }
 
 // Network-scored differential evolution: the population and the network's
@@ -190,8 +197,8 @@ int de_generation(De* d)
     Mlp* m = d->m;
     hipStream_t st = m->ctx->stream;
     ++d->gen;
-    HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));
-    TRY(forward(m, d->a.X, d->a.pop_pad, const_cast<float*>(d->a.Y)));
+    HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));   // also packs the trials into X0
+    TRY(forward_packed(m, d->a.pop_pad, const_cast<float*>(d->a.Y)));
     HIP_TRY(m->ctx, M::de_score(d->a, st));
     return MCL_OK;
 }
@@ -521,6 +528,8 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
                     hipGetErrorString(e));
     }
     a.Y = Y;
+    a.X0 = m->X0;
+    a.kp0 = m->L[0].kp;
     a.target = target;
     a.pop = p->pop;
     a.pop_pad = pad;

@@ -90,12 +90,15 @@ hipError_t optimizer(const OptArgs& a, int n_groups, hipStream_t stream);
 
 // Network-scored differential evolution (de.hip).  x [pop][dim] the population,
 // f [pop] its objectives, X [pop_pad][n_in] the network input (trials in the first
-// dim columns, the fixed inputs after them), Y [pop_pad][n_out] the network output,
+// dim columns, the fixed inputs after them; the propose kernel also writes it packed
+// into X0, as pack_input would), Y [pop_pad][n_out] the network output,
 // keys [pop] each member's (objective bits << 32 | member) after the last scoring.
 struct DeArgs {
     float* x;
     float* f;
     float* X;
+    __bf16* X0;                       // the network's packed input [pop_pad][kp0] (BF16)
+    int kp0;                          // its row length (n_in padded for the K tiling)
     const float* Y;
     const float* target;              // [n_out]
     unsigned long long* keys;
