@@ -45,7 +45,12 @@ __device__ __forceinline__ int xcd_tile(int pid, int total)
     return (pid % 8) * (total / 8) + pid / 8;
 }
 
-template <int BM, int BN, int EPI>
+// NKT = 0: two LDS stages of kBK, the next fetched while the current is
+// multiplied.  NKT > 0 (K = NKT kBK): the whole K extent of the tile is fetched at
+// once into NKT LDS stages, one barrier, then multiplied; for products too small
+// to fill the device, where each pipelined stage would wait a global-memory round
+// trip for a few matrix-core instructions.
+template <int BM, int BN, int EPI, int NKT = 0>
 __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g)
 {
     constexpr int TM = BM / 2, TN = BN / 2;       // wave tile (2 x 2 waves)
@@ -54,8 +59,9 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g)
     constexpr int CPR = kBK / 8;                  // 16-byte chunks per tile row
     constexpr int CA = BM * CPR / kThreads, CB = BN * CPR / kThreads;
     static_assert(CA >= 1 && CB >= 1, "tile too small for the loader");
-    __shared__ __align__(16) bf16 As[2][BM * LDR];
-    __shared__ __align__(16) bf16 Bs[2][BN * LDR];
+    constexpr int NBUF = NKT > 0 ? NKT : 2;
+    __shared__ __align__(16) bf16 As[NBUF][BM * LDR];
+    __shared__ __align__(16) bf16 Bs[NBUF][BN * LDR];
     __shared__ float red[kThreads / 64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -108,19 +114,55 @@ __global__ __launch_bounds__(kThreads) void gemm_kernel(GemmArgs g)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0); \
     }
 
-    // Two LDS buffers: stage kt+1 is fetched from global memory while stage kt
-    // is multiplied, then written to the other buffer; one barrier per stage.
-    const int nk = g.K / kBK;
-    MCL_GLOAD(0)
-    MCL_LSTORE(0)
-    __syncthreads();
-    for (int kt = 0; kt + 1 < nk; ++kt) {
-        MCL_GLOAD(kt + 1)
-        MCL_COMPUTE(kt & 1)
-        MCL_LSTORE((kt + 1) & 1)
+    if constexpr (NKT > 0) {
+        // every stage's global loads in flight together, then one barrier
+        uint4 sa[NKT][CA], sb[NKT][CB];
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+#pragma unroll
+            for (int j = 0; j < CA; ++j) {
+                const int c = tid + j * kThreads, r = c / CPR, q = c % CPR;
+                sa[kt][j] = *reinterpret_cast<const uint4*>(Ag + (int64_t)r * g.lda + kt * kBK + q * 8);
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int c = tid + j * kThreads, r = c / CPR, q = c % CPR;
+                sb[kt][j] = *reinterpret_cast<const uint4*>(Bg + (int64_t)r * g.ldb + kt * kBK + q * 8);
+            }
+        }
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+#pragma unroll
+            for (int j = 0; j < CA; ++j) {
+                const int c = tid + j * kThreads, r = c / CPR, q = c % CPR;
+                *reinterpret_cast<uint4*>(&As[kt][r * LDR + q * 8]) = sa[kt][j];
+            }
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
+                const int c = tid + j * kThreads, r = c / CPR, q = c % CPR;
+                *reinterpret_cast<uint4*>(&Bs[kt][r * LDR + q * 8]) = sb[kt][j];
+            }
+        }
         __syncthreads();
+#pragma unroll
+        for (int kt = 0; kt < NKT; ++kt) {
+            MCL_COMPUTE(kt)
+        }
+    } else {
+        // Two LDS buffers: stage kt+1 is fetched from global memory while stage kt
+        // is multiplied, then written to the other buffer; one barrier per stage.
+        const int nk = g.K / kBK;
+        MCL_GLOAD(0)
+        MCL_LSTORE(0)
+        __syncthreads();
+        for (int kt = 0; kt + 1 < nk; ++kt) {
+            MCL_GLOAD(kt + 1)
+            MCL_COMPUTE(kt & 1)
+            MCL_LSTORE((kt + 1) & 1)
+            __syncthreads();
+        }
+        MCL_COMPUTE((nk - 1) & 1)
     }
-    MCL_COMPUTE((nk - 1) & 1)
 #undef MCL_GLOAD
 #undef MCL_LSTORE
 #undef MCL_COMPUTE
@@ -334,7 +376,40 @@ hipError_t launch_tile(const GemmArgs& g, int tile, int splits, hipStream_t stre
     }
 }
 
+template <int EPI, int NKT>
+hipError_t launch_resident(const GemmArgs& g, hipStream_t stream)
+{
+    const int tiles = (g.M / 64) * (g.N / 64);
+    hipLaunchKernelGGL((gemm_kernel<64, 64, EPI, NKT>), dim3(tiles, 1, 1), dim3(kThreads), 0, stream, g);
+    return hipGetLastError();
+}
+
+template <int EPI>
+hipError_t resident_k(const GemmArgs& g, hipStream_t stream)
+{
+    switch (g.K / kBK) {
+    case 1: return launch_resident<EPI, 1>(g, stream);
+    case 2: return launch_resident<EPI, 2>(g, stream);
+    case 4: return launch_resident<EPI, 4>(g, stream);
+    case 8: return launch_resident<EPI, 8>(g, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
+
+bool gemm_resident_ok(const GemmArgs& g, int tile, int epi, int splits)
+{
+    const int nk = g.K / kBK;
+    return tile == kTile64x64 && splits == 1 && g.K % kBK == 0 &&
+           (nk == 1 || nk == 2 || nk == 4 || nk == 8) && (epi == kHiddenInfer || epi == kOutInfer);
+}
+
+hipError_t gemm_resident(const GemmArgs& g, int epi, hipStream_t stream)
+{
+    return epi == kHiddenInfer ? resident_k<kHiddenInfer>(g, stream)
+                               : resident_k<kOutInfer>(g, stream);
+}
 
 hipError_t gemm(const GemmArgs& g, int tile, int epi, int splits, hipStream_t stream)
 {

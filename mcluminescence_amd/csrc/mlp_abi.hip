@@ -95,7 +95,10 @@ int gemm(Mlp* m, const M::GemmArgs& g, int epi, int splits = 1)
 {
     const int tile = pick_tile(m, g.M, g.N);
     if (tile < 0) return fail(m->ctx, MCL_EINVAL, "no tile shape divides a %d x %d product", g.M, g.N);
-    HIP_TRY(m->ctx, M::gemm(g, tile, epi, splits, m->ctx->stream));
+    if (M::gemm_resident_ok(g, tile, epi, splits))
+        HIP_TRY(m->ctx, M::gemm_resident(g, epi, m->ctx->stream));
+    else
+        HIP_TRY(m->ctx, M::gemm(g, tile, epi, splits, m->ctx->stream));
     return MCL_OK;
 }
 

@@ -46,6 +46,10 @@ struct GemmArgs {
 };
 
 hipError_t gemm(const GemmArgs& g, int tile, int epi, int splits, hipStream_t stream);
+// The whole-K variant for products too small to fill the device: 64 x 64 tiles,
+// K = 64, 128, 256 or 512 held in LDS at once, inference epilogues only.
+bool gemm_resident_ok(const GemmArgs& g, int tile, int epi, int splits);
+hipError_t gemm_resident(const GemmArgs& g, int epi, hipStream_t stream);
 
 // x [rows][n_in] FP32 (row idx[b], or b) -> X [batch][kp] BF16 zero-padded and,
 // if xt is given, X^T [kp][batch].
