@@ -99,6 +99,8 @@ typedef struct mcl_stats {
 /* ---- context ------------------------------------------------------------- */
 int         mcl_abi_version(void);
 int         mcl_create(void** ctx_out, int hip_device);
+/* A context keeps the device buffers of finished runs and destroyed plans (up to
+ * 512 MiB) for later ones; mcl_destroy frees them. */
 void        mcl_destroy(void* ctx);
 const char* mcl_last_error(const void* ctx);
 /* Launch subsequent work on an existing HIP stream (e.g. torch's current

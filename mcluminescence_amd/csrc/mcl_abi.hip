@@ -20,6 +20,69 @@
 #include "mcl_ctx.h"
 #include "mcl_internal.h"
 
+namespace mcl {
+
+namespace {
+constexpr size_t kPoolCap = size_t(512) << 20;   // idle bytes a context keeps at most
+}
+
+hipError_t pool_alloc(Ctx* c, void** p, size_t bytes)
+{
+    bytes = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+    {
+        std::lock_guard<std::mutex> lock(c->pool_mu);
+        // the smallest idle block that fits and wastes at most about half of itself
+        size_t best = c->pool_free.size();
+        for (size_t i = 0; i < c->pool_free.size(); ++i) {
+            const size_t b = c->pool_free[i].first;
+            if (b >= bytes && b <= 2 * bytes + 65536 &&
+                (best == c->pool_free.size() || b < c->pool_free[best].first))
+                best = i;
+        }
+        if (best < c->pool_free.size()) {
+            const auto blk = c->pool_free[best];
+            c->pool_free[best] = c->pool_free.back();
+            c->pool_free.pop_back();
+            c->pool_cached -= blk.first;
+            c->pool_live[blk.second] = blk.first;
+            *p = blk.second;
+            return hipSuccess;
+        }
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(c->pool_mu);
+    c->pool_live[*p] = bytes;
+    return hipSuccess;
+}
+
+void pool_release(Ctx* c, void* p, bool idle)
+{
+    if (!p) return;
+    if (!idle) (void)hipDeviceSynchronize();
+    std::lock_guard<std::mutex> lock(c->pool_mu);
+    const auto it = c->pool_live.find(p);
+    if (it == c->pool_live.end()) return;
+    const size_t bytes = it->second;
+    c->pool_live.erase(it);
+    if (c->pool_cached + bytes > kPoolCap) {
+        (void)hipFree(p);
+        return;
+    }
+    c->pool_free.emplace_back(bytes, p);
+    c->pool_cached += bytes;
+}
+
+void pool_clear(Ctx* c)
+{
+    std::lock_guard<std::mutex> lock(c->pool_mu);
+    for (const auto& blk : c->pool_free) (void)hipFree(blk.second);
+    c->pool_free.clear();
+    c->pool_cached = 0;
+}
+
+}  // namespace mcl
+
 namespace {
 
 using mcl::Ctx;
@@ -162,15 +225,18 @@ mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_i
     return d;
 }
 
-void plan_free(Plan* p)
+// idle: no kernel or copy of the plan is still running (its stream was synchronised).
+void plan_free(Plan* p, bool idle)
 {
     if (!p) return;
-    hipFree(p->d_runs);
-    hipFree(p->d_crun);
-    hipFree(p->d_csim);
-    hipFree(p->d_work);
-    hipFree(p->d_order);
-    hipFree(p->d_tickets);
+    Ctx* c = p->ctx;
+    if (!idle) (void)hipDeviceSynchronize();
+    mcl::pool_release(c, p->d_runs, true);
+    mcl::pool_release(c, p->d_crun, true);
+    mcl::pool_release(c, p->d_csim, true);
+    mcl::pool_release(c, p->d_work, true);
+    mcl::pool_release(c, p->d_order, true);
+    mcl::pool_release(c, p->d_tickets, true);
     if (p->gexec) hipGraphExecDestroy(p->gexec);
     for (hipStream_t s : p->side) hipStreamDestroy(s);
     for (hipEvent_t e : p->join) hipEventDestroy(e);
@@ -184,8 +250,9 @@ void plan_free(Plan* p)
 // need lds bytes each: the tuning knob when set, otherwise the one of 1, 2, 4, 8
 // (at most max_waves) that keeps the most waves resident (clusters resident at
 // once, from the occupancy of that kernel, times waves), the fewest on a tie.  A
-// wave gets at least one 64-slot row at the start, and eight waves need two rows
-// each: with one, four waves measured as fast.  Size classes that run side by side
+// wave gets at least one 64-slot row at the start (with 8-15 rows eight waves
+// measured 2-20 % faster than four on launches of 16-64 clusters; with 5, four are
+// 20 % faster than two).  Size classes that run side by side
 // take at most four, so that a class of large clusters leaves wave slots on its
 // CUs to the others (eight measured 20 % slower on an uneven sweep).  per_cu:
 // that kernel's workgroups per CU.
@@ -198,7 +265,7 @@ int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int max_waves, int* 
     int64_t best_res = -1;
     for (int w : {1, 2, 4, 8}) {
         if (forced != 0 && w != forced) continue;
-        if (forced == 0 && (w > max_waves || w > rows || (w == 8 && rows < 16))) break;
+        if (forced == 0 && (w > max_waves || w > rows)) break;
         if (lds + mcl::fixed_dt_waves_lds(w) > c->max_lds) continue;
         int pc = 0;
         const hipError_t e = mcl::fixed_dt_occupancy(lds, w, &pc);
@@ -276,8 +343,9 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
         cl.grid = std::min(grid, cl.count);
     }
     const size_t ncls = p->classes.size();
-    hipError_t e = hipMalloc((void**)&p->d_order, sizeof(int32_t) * order.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_tickets, sizeof(uint32_t) * std::max<size_t>(ncls, 1));
+    hipError_t e = mcl::pool_alloc(c, (void**)&p->d_order, sizeof(int32_t) * order.size());
+    if (e == hipSuccess)
+        e = mcl::pool_alloc(c, (void**)&p->d_tickets, sizeof(uint32_t) * std::max<size_t>(ncls, 1));
     if (e == hipSuccess)
         e = hipMemcpy(p->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
@@ -351,10 +419,11 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->max_e = max_e;
     p->max_h = max_h;
     p->flags = flags;
-    hipError_t e = hipMalloc((void**)&p->d_runs, sizeof(mcl::RunDesc) * (size_t)n_runs);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_crun, sizeof(int32_t) * (size_t)total);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_csim, sizeof(int32_t) * (size_t)total);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_work, sizeof(uint64_t) * 4 * (size_t)total);
+    hipError_t e = mcl::pool_alloc(c, (void**)&p->d_runs, sizeof(mcl::RunDesc) * (size_t)n_runs);
+    if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_crun, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_csim, sizeof(int32_t) * (size_t)total);
+    if (e == hipSuccess)
+        e = mcl::pool_alloc(c, (void**)&p->d_work, sizeof(uint64_t) * 4 * (size_t)total);
     if (e == hipSuccess)
         e = hipMemcpyAsync(p->d_runs, desc.data(), sizeof(mcl::RunDesc) * (size_t)n_runs,
                            hipMemcpyHostToDevice, c->stream);
@@ -369,7 +438,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (e == hipSuccess) e = hipEventCreate(&p->ev1);
     if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds, c->max_lds);
     if (e != hipSuccess) {
-        plan_free(p);
+        plan_free(p, false);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "plan setup: %s",
                     hipGetErrorString(e));
     }
@@ -377,14 +446,14 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
         int per_cu = 0;
         const int st = choose_waves(c, lds, e_cap, total, 8, &p->waves, &per_cu);
         if (st) {
-            plan_free(p);
+            plan_free(p, true);
             return st;
         }
     }
     if (flags & (MCL_F_PERSISTENT | MCL_F_GRAPH)) {
         const int st = build_classes(c, p, runs, n_runs, desc);
         if (st) {
-            plan_free(p);
+            plan_free(p, true);
             return st;
         }
     }
@@ -529,6 +598,7 @@ void mcl_destroy(void* ctx)
         hipStreamSynchronize(c->own_stream);
         hipStreamDestroy(c->own_stream);
     }
+    mcl::pool_clear(c);
     delete c;
 }
 
@@ -701,7 +771,7 @@ int mcl_plan_work(void* plan, mcl_work* out)
     return MCL_OK;
 }
 
-void mcl_plan_destroy(void* plan) { plan_free((Plan*)plan); }
+void mcl_plan_destroy(void* plan) { plan_free((Plan*)plan, false); }
 
 int mcl_sync(void* ctx)
 {
@@ -785,12 +855,14 @@ int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const dou
     }
     std::vector<void*> tmp;
     auto dmalloc = [&](void** ptr, size_t bytes) {
-        const hipError_t e = hipMalloc(ptr, bytes);
+        const hipError_t e = mcl::pool_alloc(c, ptr, bytes);
         if (e == hipSuccess) tmp.push_back(*ptr);
         return e;
     };
-    auto release = [&]() {
-        for (void* q : tmp) (void)hipFree(q);
+    // idle: the stream was synchronised after the last use (else a device sync first)
+    auto release = [&](bool idle) {
+        if (!idle) (void)hipDeviceSynchronize();
+        for (void* q : tmp) mcl::pool_release(c, q, true);
     };
     mcl::RunDesc* d_runs = nullptr;
     int32_t *d_crun = nullptr, *d_csim = nullptr, *d_off = nullptr, *d_cnt = nullptr,
@@ -808,7 +880,7 @@ int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const dou
     if (e == hipSuccess) e = dmalloc((void**)&d_ratio, sizeof(double) * n_ratio);
     if (e == hipSuccess) e = dmalloc((void**)&d_work, sizeof(uint64_t) * 4 * (size_t)total);
     if (e != hipSuccess) {
-        release();
+        release(true);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "allocation: %s",
                     hipGetErrorString(e));
     }
@@ -861,7 +933,7 @@ int mcl_run_lab(void* ctx, const mcl_run_params* runs, int32_t n_runs, const dou
     if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ev0, ev1);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    release();
+    release(e == hipSuccess);
     if (e != hipSuccess)
         return fail(c, MCL_EDEVICE, "lab kernel or copy failed: %s", hipGetErrorString(e));
     if (status) std::memcpy(status, hs.data(), sizeof(int32_t) * (size_t)total);
@@ -943,12 +1015,14 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     const size_t rowsz = (size_t)total * (size_t)max_steps;
     std::vector<void*> tmp;
     auto dmalloc = [&](void** ptr, size_t bytes) {
-        const hipError_t e = hipMalloc(ptr, bytes);
+        const hipError_t e = mcl::pool_alloc(c, ptr, bytes);
         if (e == hipSuccess) tmp.push_back(*ptr);
         return e;
     };
-    auto release = [&]() {
-        for (void* q : tmp) (void)hipFree(q);
+    // idle: the stream was synchronised after the last use (else a device sync first)
+    auto release = [&](bool idle) {
+        if (!idle) (void)hipDeviceSynchronize();
+        for (void* q : tmp) mcl::pool_release(c, q, true);
     };
     mcl::RunDesc* d_runs = nullptr;
     int32_t *d_crun = nullptr, *d_csim = nullptr, *d_steps = nullptr;
@@ -965,7 +1039,7 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     if (e == hipSuccess && !dev && t_axis) e = dmalloc((void**)&d_t, sizeof(double) * rowsz);
     if (e == hipSuccess && !dev && e_ratio) e = dmalloc((void**)&d_er, sizeof(double) * rowsz);
     if (e != hipSuccess) {
-        release();
+        release(true);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "allocation: %s",
                     hipGetErrorString(e));
     }
@@ -1019,7 +1093,7 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ev0, ev1);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    release();
+    release(e == hipSuccess);
     if (e != hipSuccess) return fail(c, MCL_EDEVICE, "event kernel or copy failed: %s",
                                      hipGetErrorString(e));
     std::memcpy(steps, hs.data(), sizeof(int32_t) * (size_t)total);
@@ -1063,11 +1137,11 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
     uint32_t* d_counts = dev ? counts : nullptr;
     uint64_t* d_live = nullptr;
     hipError_t e = hipSuccess;
-    if (!dev) e = hipMalloc((void**)&d_counts, sizeof(uint32_t) * n_counts);
-    if (e == hipSuccess) e = hipMalloc((void**)&d_live, sizeof(uint64_t) * (size_t)p->total);
+    if (!dev) e = mcl::pool_alloc(c, (void**)&d_counts, sizeof(uint32_t) * n_counts);
+    if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&d_live, sizeof(uint64_t) * (size_t)p->total);
     if (e != hipSuccess) {
-        if (!dev) hipFree(d_counts);
-        plan_free(p);
+        if (!dev) mcl::pool_release(c, d_counts, true);
+        plan_free(p, true);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "output allocation: %s",
                     hipGetErrorString(e));
     }
@@ -1102,9 +1176,11 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
             stats->cluster_steps = w.cluster_steps;
         }
     }
-    if (!dev) hipFree(d_counts);
-    hipFree(d_live);
-    plan_free(p);
+    // after a failed launch or copy kernels may still run: release after a device sync
+    if (e != hipSuccess || st != MCL_OK) (void)hipDeviceSynchronize();
+    if (!dev) mcl::pool_release(c, d_counts, true);
+    mcl::pool_release(c, d_live, true);
+    plan_free(p, true);
     return st;
 }
 
@@ -1137,7 +1213,7 @@ int mcl_cluster_state(void* ctx, const mcl_run_params* run, int32_t run_index, i
     const size_t bytes = ne * 12 + nh * 12 + ne + nh + ne * 4 + ne * 4 + 64 + 64;
     if (e == hipSuccess) e = hipMalloc((void**)&buf, bytes);
     if (e != hipSuccess) {
-        plan_free(p);
+        plan_free(p, false);
         return fail(c, MCL_EDEVICE, "cluster_state setup: %s", hipGetErrorString(e));
     }
     mcl::StateDump d{};
@@ -1163,7 +1239,7 @@ int mcl_cluster_state(void* ctx, const mcl_run_params* run, int32_t run_index, i
         if (e != hipSuccess) st = fail(c, MCL_EDEVICE, "cluster_state: %s", hipGetErrorString(e));
     }
     hipFree(buf);
-    plan_free(p);
+    plan_free(p, false);
     return st;
 }
 

@@ -6,7 +6,11 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
 
 #include "../../include/mcl.h"
 
@@ -24,7 +28,22 @@ struct Ctx {
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
     double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
     std::string err;
+    // Device blocks of finished runs, kept for later runs (hipMalloc and hipFree,
+    // which waits for the device, cost more than a small run's kernels).
+    std::mutex pool_mu;
+    std::vector<std::pair<size_t, void*>> pool_free;   // (bytes, block), idle
+    std::unordered_map<void*, size_t> pool_live;        // handed out: block -> bytes
+    size_t pool_cached = 0;                             // bytes in pool_free
 };
+
+// A device block of at least `bytes` from the context's pool, else hipMalloc.
+hipError_t pool_alloc(Ctx* c, void** p, size_t bytes);
+// Returns a block to the pool.  idle: every kernel and copy that used it has
+// finished (the caller synchronised the stream it ran on); otherwise the device
+// is synchronised first, as hipFree would.
+void pool_release(Ctx* c, void* p, bool idle);
+// Frees the idle blocks (at context destruction).
+void pool_clear(Ctx* c);
 
 // Record a formatted message as the context's last error and return code.
 inline int fail(Ctx* c, int code, const char* fmt, ...)

@@ -123,15 +123,17 @@ def tl_table(theta: np.ndarray, n: int, sims: int) -> np.ndarray:
 def mc_curves(ctx: Context, theta: np.ndarray, n: int = 400, sims: int = 8,
               seed: int = DEFAULT_SEED, chunk: int = 8192) -> np.ndarray:
     """Monte Carlo glow curves [N, 256]: recombinations per bin summed over the
-    sims, divided by sims * N_e.  The parameter sets are uneven in work, so each chunk
-    runs as one persistent launch (clusters taken largest expected work first; the
-    counts equal those of a one-workgroup-per-cluster launch)."""
+    sims, divided by sims * N_e.  Parameter sets are uneven in work, so a chunk of
+    several runs as one persistent launch (clusters taken largest expected work
+    first); one parameter set (equal clusters) runs one workgroup per cluster, with
+    no graph to build.  The counts are the same either way."""
     theta = np.atleast_2d(theta)
     out = np.empty((theta.shape[0], N_BINS), np.float64)
     for c0 in range(0, theta.shape[0], chunk):
         part = theta[c0:c0 + chunk]
         runs = tl_table(part, n, sims)
-        counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=F_PERSISTENT)
+        flags = F_PERSISTENT if len(part) > 1 else 0
+        counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=flags)
         out[c0:c0 + len(part)] = counts.reshape(len(part), sims, N_BINS).sum(axis=1) / (sims * n)
     return out
 
