@@ -102,10 +102,18 @@ typedef struct mcl_de_params {
     uint64_t seed;            /* Philox key                                                */
     float fixed[8];           /* inputs dim .. n_in-1 (n_in - dim <= 8)                    */
     float log_span, out_scale, inv_peak2;
+    int32_t flags;            /* MCL_DE_LAYERED: a generation as separate kernels          */
 } mcl_de_params;
 
+/* mcl_de_params.flags: run each generation as a propose kernel, the network's
+ * per-layer GEMM kernels and a score kernel instead of one fused kernel (the
+ * results are identical; the fused kernel needs layers at most 512 wide). */
+#define MCL_DE_LAYERED 1
+
 /* Creates the search and scores its initial population (uniform in [-1, 1]);
- * target: DEVICE [n_out], read by every generation. */
+ * target: DEVICE [n_out], read by every generation.  The fused generation kernel
+ * reads a copy of the network's weights made here: training the network during
+ * the search does not change what the search sees. */
 int  mcl_de_create(void* mlp, const mcl_de_params* params, const float* target, void** de_out);
 void mcl_de_destroy(void* de);
 /* Runs gens generations on the context's stream, without a host synchronisation. */

@@ -119,7 +119,10 @@ class DeParams(C.Structure):
     """mcl_de_params (include/mcl_surrogate.h)."""
     _fields_ = [("pop", C.c_int32), ("dim", C.c_int32), ("F", C.c_float), ("CR", C.c_float),
                 ("seed", C.c_uint64), ("fixed", C.c_float * 8), ("log_span", C.c_float),
-                ("out_scale", C.c_float), ("inv_peak2", C.c_float)]
+                ("out_scale", C.c_float), ("inv_peak2", C.c_float), ("flags", C.c_int32)]
+
+
+DE_LAYERED = 1   # mcl_de_params.flags: a generation as separate kernels
 
 
 class MclError(RuntimeError):

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <new>
 #include <vector>
 
@@ -180,12 +181,19 @@ int forward(Mlp* m, const float* x, int32_t B, float* y)
     return forward_packed(m, B, y);
 }
 
-// Network-scored differential evolution: the population and the network's
-// input / output rows of one generation, in device memory.
+// Network-scored differential evolution: the population (two buffers: the one a
+// generation reads and the one its selection writes) and the network's input /
+// output rows of one generation, in device memory.
 struct De {
     Mlp* m = nullptr;
     M::DeArgs a{};
+    float* xb[2] = {nullptr, nullptr};                  // [pop][dim]
+    unsigned long long* kb[2] = {nullptr, nullptr};     // argmin keys [pop]
+    int cur = 0;                          // the buffer holding the population
     uint32_t gen = 0;                     // generations run (0: the initial population)
+    bool fused = false;                   // one kernel per generation (de_generation_kernel)
+    M::DeNet net{};
+    int fused_lds = 0;
     std::vector<void*> allocs;
 
     ~De()
@@ -194,15 +202,72 @@ struct De {
     }
 };
 
+// Points the arguments at the current population and the one selection writes.
+void de_buffers(De* d)
+{
+    d->a.x = d->xb[d->cur];
+    d->a.x_next = d->xb[d->cur ^ 1];
+    d->a.keys = d->kb[d->cur];
+    d->a.keys_next = d->kb[d->cur ^ 1];
+}
+
 // One generation on the context's stream: trials, their network curves, selection.
 int de_generation(De* d)
 {
     Mlp* m = d->m;
     hipStream_t st = m->ctx->stream;
     ++d->gen;
-    HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));   // also packs the trials into X0
-    TRY(forward_packed(m, d->a.pop_pad, const_cast<float*>(d->a.Y)));
-    HIP_TRY(m->ctx, M::de_score(d->a, st));
+    de_buffers(d);
+    if (d->fused) {
+        HIP_TRY(m->ctx, M::de_generation_fused(d->a, d->net, d->gen, d->fused_lds, st));
+    } else {
+        HIP_TRY(m->ctx, M::de_propose(d->a, d->gen, st));   // also packs the trials into X0
+        TRY(forward_packed(m, d->a.pop_pad, const_cast<float*>(d->a.Y)));
+        HIP_TRY(m->ctx, M::de_score(d->a, st));
+    }
+    d->cur ^= 1;
+    return MCL_OK;
+}
+
+// Whether the fused generation kernel takes the network (every layer at most 512
+// outputs, K a multiple of 64 up to 512, its LDS within a workgroup's); fills
+// net with the layers' shapes and biases (weights: set by de_fused_weights).
+bool de_fused_net(const Mlp* m, M::DeNet* net, int* lds)
+{
+    if (m->L.size() > (size_t)M::kMaxLayers) return false;
+    int widest = 0;
+    for (size_t l = 0; l < m->L.size(); ++l) {
+        const Layer& s = m->L[l];
+        if (s.out % 32 || s.out > 512 || s.kp % 64 || s.kp > 512) return false;
+        net->L[l] = M::DeLayer{nullptr, s.bw, s.out, s.kp};
+        widest = std::max(widest, s.kp);
+        if (l + 1 < m->L.size()) widest = std::max(widest, s.out);
+    }
+    net->n_layers = (int)m->L.size();
+    net->lda = widest + 8;
+    *lds = M::de_fused_lds(*net, m->d.n_out);
+    return *lds <= m->ctx->max_lds;
+}
+
+// The fused kernel's copy of the weights in fragment order (one buffer for all
+// layers), packed on the context's stream from the network's current weights.
+int de_fused_weights(De* d, const std::function<hipError_t(void**, size_t)>& alloc)
+{
+    Mlp* m = d->m;
+    size_t total = 0;
+    for (const Layer& s : m->L) total += (size_t)s.out * s.kp;
+    bf16* wf = nullptr;
+    const hipError_t e = alloc((void**)&wf, total * sizeof(bf16));
+    if (e != hipSuccess)
+        return fail(m->ctx, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE,
+                    "fused DE weights: %s", hipGetErrorString(e));
+    size_t off = 0;
+    for (size_t l = 0; l < m->L.size(); ++l) {
+        const Layer& s = m->L[l];
+        HIP_TRY(m->ctx, M::de_pack(s.wb, s.out, s.kp, wf + off, m->ctx->stream));
+        d->net.L[l].w = wf + off;
+        off += (size_t)s.out * s.kp;
+    }
     return MCL_OK;
 }
 
@@ -520,11 +585,14 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
     };
     M::DeArgs& a = d->a;
     float* Y = nullptr;
-    hipError_t e = alloc((void**)&a.x, sizeof(float) * (size_t)p->pop * p->dim);
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = alloc((void**)&d->xb[b], sizeof(float) * (size_t)p->pop * p->dim);
+        if (e == hipSuccess) e = alloc((void**)&d->kb[b], sizeof(unsigned long long) * (size_t)p->pop);
+    }
     if (e == hipSuccess) e = alloc((void**)&a.f, sizeof(float) * (size_t)p->pop);
     if (e == hipSuccess) e = alloc((void**)&a.X, sizeof(float) * (size_t)pad * n_in);
     if (e == hipSuccess) e = alloc((void**)&Y, sizeof(float) * (size_t)pad * n_out);
-    if (e == hipSuccess) e = alloc((void**)&a.keys, sizeof(unsigned long long) * (size_t)p->pop);
     if (e != hipSuccess) {
         delete d;
         return fail(m->ctx, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "DE buffers: %s",
@@ -547,13 +615,19 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
     a.key0 = (uint32_t)p->seed;
     a.key1 = (uint32_t)(p->seed >> 32);
     for (int k = 0; k < 8; ++k) a.fixed[k] = k < n_in - p->dim ? p->fixed[k] : 0.0f;
+    d->fused = !(p->flags & MCL_DE_LAYERED) && de_fused_net(m, &d->net, &d->fused_lds);
     // generation 0: the initial population, uniform in [-1, 1], scored
     hipStream_t st = m->ctx->stream;
     int rc = MCL_OK;
-    if (M::de_init(a, st) != hipSuccess) rc = fail(m->ctx, MCL_EDEVICE, "de_init launch");
+    if (d->fused && M::de_fused_prepare(d->fused_lds) != hipSuccess)
+        rc = fail(m->ctx, MCL_EDEVICE, "fused generation kernel: LDS attribute");
+    if (rc == MCL_OK && d->fused) rc = de_fused_weights(d, alloc);
+    de_buffers(d);
+    if (rc == MCL_OK && M::de_init(a, st) != hipSuccess) rc = fail(m->ctx, MCL_EDEVICE, "de_init launch");
     if (rc == MCL_OK) rc = forward(m, a.X, pad, Y);
     if (rc == MCL_OK && M::de_score(a, st) != hipSuccess)
         rc = fail(m->ctx, MCL_EDEVICE, "de_score launch");
+    d->cur ^= 1;
     if (rc != MCL_OK) {
         (void)hipStreamSynchronize(st);
         delete d;
@@ -593,7 +667,7 @@ int mcl_de_best(void* de, float* x, float* f)
     int i = 0;
     for (int j = 1; j < d->a.pop; ++j)
         if (fh[(size_t)j] < fh[(size_t)i] || (std::isnan(fh[(size_t)i]) && !std::isnan(fh[(size_t)j]))) i = j;
-    HIP_TRY(c, hipMemcpyAsync(x, d->a.x + (size_t)i * d->a.dim, sizeof(float) * d->a.dim,
+    HIP_TRY(c, hipMemcpyAsync(x, d->xb[d->cur] + (size_t)i * d->a.dim, sizeof(float) * d->a.dim,
                               hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(f, d->a.f + i, sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -605,7 +679,7 @@ int mcl_de_population(void* de, float* x, float* f)
     De* d = static_cast<De*>(de);
     if (!d || !x || !f) return MCL_EINVAL;
     Ctx* c = d->m->ctx;
-    HIP_TRY(c, hipMemcpyAsync(x, d->a.x, sizeof(float) * (size_t)d->a.pop * d->a.dim,
+    HIP_TRY(c, hipMemcpyAsync(x, d->xb[d->cur], sizeof(float) * (size_t)d->a.pop * d->a.dim,
                               hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(f, d->a.f, sizeof(float) * (size_t)d->a.pop,
                               hipMemcpyDeviceToDevice, c->stream));

@@ -98,14 +98,16 @@ hipError_t optimizer(const OptArgs& a, int n_groups, hipStream_t stream);
 // into X0, as pack_input would), Y [pop_pad][n_out] the network output,
 // keys [pop] each member's (objective bits << 32 | member) after the last scoring.
 struct DeArgs {
-    float* x;
-    float* f;
+    float* x;                         // members read by this generation [pop][dim]
+    float* x_next;                    // members after its selection
+    float* f;                         // objectives [pop] (member i's only by its own selection)
     float* X;
     __bf16* X0;                       // the network's packed input [pop_pad][kp0] (BF16)
     int kp0;                          // its row length (n_in padded for the K tiling)
     const float* Y;
     const float* target;              // [n_out]
-    unsigned long long* keys;
+    const unsigned long long* keys;   // argmin keys of x [pop]
+    unsigned long long* keys_next;    // of x_next
     int pop, pop_pad, dim, n_in, n_out;
     float F, CR;
     float log_span, out_scale, inv_peak2;
@@ -115,6 +117,25 @@ struct DeArgs {
 hipError_t de_init(const DeArgs& a, hipStream_t stream);
 hipError_t de_propose(const DeArgs& a, uint32_t gen, hipStream_t stream);
 hipError_t de_score(const DeArgs& a, hipStream_t stream);
+
+// The network as the fused generation kernel reads it: per layer the BF16 weights
+// in fragment order (de_pack of the [out][kp] weights) and the FP32 bias; lda =
+// the LDS row stride of its activation buffers (the widest kp or hidden width,
+// plus 8).
+struct DeLayer {
+    const __bf16* w;
+    const float* b;
+    int out, kp;
+};
+struct DeNet {
+    DeLayer L[kMaxLayers];
+    int n_layers, lda;
+};
+hipError_t de_pack(const __bf16* W, int out, int kp, __bf16* Wf, hipStream_t stream);
+int de_fused_lds(const DeNet& net, int n_out);
+hipError_t de_fused_prepare(int lds_bytes);
+hipError_t de_generation_fused(const DeArgs& a, const DeNet& net, uint32_t gen, int lds_bytes,
+                               hipStream_t stream);
 
 }  // namespace mlp
 }  // namespace mcl

@@ -114,15 +114,18 @@ class NetworkDE:
     """The network-scored search of the surrogate-assisted fit, in device memory
     (mcl_de_*, include/mcl_surrogate.h): the DE/rand-to-best/1/bin step of
     DifferentialEvolution, with the population, the trials and the network's
-    curves on the GPU.  A generation is a propose kernel, the network forward and
-    a score-and-accept kernel; run(k) queues k generations with no host round
-    trip.  The objective is objective() of the network's de-normalised curve:
-    mean over bins of (curve - y)^2 / peak(y)^2."""
+    curves on the GPU.  A generation is one kernel (trials, the network forward in
+    LDS, objectives and selection), or with layered=True a propose kernel, the
+    network's per-layer kernels and a score-and-accept kernel, with identical
+    results; run(k) queues k generations with no host round trip.  The objective
+    is objective() of the network's de-normalised curve: mean over bins of
+    (curve - y)^2 / peak(y)^2."""
 
     def __init__(self, net: S.Network, y: np.ndarray, scale: float, fixed: Sequence[float],
-                 dim: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3):
+                 dim: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3,
+                 layered: bool = False):
         import torch
-        from ._abi import DeParams
+        from ._abi import DE_LAYERED, DeParams
         self.net, self.dim, self.pop = net, dim, pop
         self.target = torch.from_numpy(np.asarray(y, np.float32)).cuda()
         p = DeParams()
@@ -132,6 +135,7 @@ class NetworkDE:
         p.log_span = math.log1p(1.0 / net.c0)
         p.out_scale = net.c0 * scale
         p.inv_peak2 = 1.0 / max(float(np.max(y)), 1.0) ** 2
+        p.flags = DE_LAYERED if layered else 0
         h = C.c_void_p()
         with net._on_stream(self.target):
             net.ctx.check(net.lib.mcl_de_create(net.handle, C.byref(p), self.target.data_ptr(),

@@ -1,7 +1,8 @@
 """The network-scored differential evolution in device memory (mcl_de_*): its
 objective against the same objective computed by torch from the network's
 forward, greedy selection, the argmin, determinism, convergence on a target the
-network can reproduce exactly, and argument checks."""
+network can reproduce exactly, the fused one-kernel generation against the
+layered one (bit-identical populations), and argument checks."""
 import math
 
 import numpy as np
@@ -87,6 +88,27 @@ def test_search_is_deterministic_and_converges(net):
     np.testing.assert_array_equal(xa, xb)
     assert fa == fb
     assert fa < 1e-3 * f_start and fa < 1e-4, (fa, f_start)
+
+
+@pytest.mark.parametrize("pop", [300, 1024])
+def test_fused_generation_matches_layered(net, pop):
+    print(f"DE: fused vs layered generations, 40 generations (pop {pop})", flush=True)
+    import torch
+    tgt = _target(net, [-0.1, 0.25, 0.6, -0.45, 0.3]).cpu().numpy()
+    res = []
+    for layered in (False, True):
+        de = F.NetworkDE(net, tgt, SCALE, [U_BETA], 5, pop, seed=21, layered=layered)
+        try:
+            de.run(15)
+            de.run(25)
+            x, f = de.population()
+            res.append((x.clone(), f.clone(), de.best()))
+        finally:
+            de.close()
+    (xa, fa, ba), (xb, fb, bb) = res
+    assert torch.equal(xa, xb) and torch.equal(fa, fb)
+    np.testing.assert_array_equal(ba[0], bb[0])
+    assert ba[1] == bb[1]
 
 
 def test_invalid_parameters_are_rejected(net):
