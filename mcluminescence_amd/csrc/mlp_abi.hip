@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <new>
 #include <vector>
@@ -659,18 +660,22 @@ int mcl_de_best(void* de, float* x, float* f)
     De* d = static_cast<De*>(de);
     if (!d || !x || !f) return MCL_EINVAL;
     Ctx* c = d->m->ctx;
-    std::vector<float> fh((size_t)d->a.pop);
-    HIP_TRY(c, hipMemcpyAsync(fh.data(), d->a.f, sizeof(float) * fh.size(), hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    // first index of the minimum, as the kernels' argmin (unordered objectives last)
-    int i = 0;
-    for (int j = 1; j < d->a.pop; ++j)
-        if (fh[(size_t)j] < fh[(size_t)i] || (std::isnan(fh[(size_t)i]) && !std::isnan(fh[(size_t)j]))) i = j;
-    HIP_TRY(c, hipMemcpyAsync(x, d->xb[d->cur] + (size_t)i * d->a.dim, sizeof(float) * d->a.dim,
+    // the members' argmin keys (objective bits << 32 | member: the objectives are
+    // >= 0, so the smallest key is the first index of the minimum, unordered
+    // objectives last) and the population, with one synchronisation
+    const size_t pop = (size_t)d->a.pop, dim = (size_t)d->a.dim;
+    std::vector<unsigned long long> kh(pop);
+    std::vector<float> xh(pop * dim);
+    HIP_TRY(c, hipMemcpyAsync(kh.data(), d->kb[d->cur], sizeof(unsigned long long) * pop,
                               hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(f, d->a.f + i, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(xh.data(), d->xb[d->cur], sizeof(float) * pop * dim,
+                              hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const unsigned long long k = *std::min_element(kh.begin(), kh.end());
+    const size_t i = (size_t)(uint32_t)k;
+    const uint32_t fb = (uint32_t)(k >> 32);
+    std::memcpy(f, &fb, sizeof fb);
+    std::memcpy(x, xh.data() + i * dim, sizeof(float) * dim);
     return MCL_OK;
 }
 
