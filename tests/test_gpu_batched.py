@@ -5,7 +5,7 @@ import pytest
 
 from mcluminescence_amd import autotune
 from mcluminescence_amd import workloads as W
-from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT
+from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT, Context, RunParams
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -74,3 +74,28 @@ def test_autotuner_picks_a_measured_point(gpu_ctx, tmp_path):
     knobs = autotune.apply(gpu_ctx, runs, table)
     assert knobs["holes_per_cell"] == entry["holes_per_cell"]
     gpu_ctx.tune_set("holes_per_cell", 2.0)
+
+
+def test_reused_device_buffers_give_fresh_context_results(gpu_ctx):
+    """Runs take their device buffers from the context's pool: a run that reuses the
+    blocks of larger runs (fixed-dt, persistent and event mode) gives exactly what a
+    fresh context gives."""
+    print("device block reuse: sweep 240 sets, then 80 sets, then an event run", flush=True)
+    big, small = _sweep(), _sweep()[:80]
+    ev = RunParams.make(**{**W.tunnelling_tl12(sims=1, n=200)[0].as_dict(), "sims": 3, "n_steps": 1})
+    ev.dt = 1.0
+    gpu_ctx.run_fixed_dt(big, n_bins=48)                       # leaves its blocks in the pool
+    gpu_ctx.run_fixed_dt(big, n_bins=48, flags=F_PERSISTENT)
+    got = [gpu_ctx.run_fixed_dt(small, n_bins=48)[:2],
+           gpu_ctx.run_fixed_dt(small, n_bins=48, flags=F_PERSISTENT)[:2],
+           gpu_ctx.run_event([ev], [800.0], 1200)[:4]]
+    fresh = Context(0)
+    try:
+        want = [fresh.run_fixed_dt(small, n_bins=48)[:2],
+                fresh.run_fixed_dt(small, n_bins=48, flags=F_PERSISTENT)[:2],
+                fresh.run_event([ev], [800.0], 1200)[:4]]
+    finally:
+        fresh.close()
+    for g, w in zip(got, want):
+        for a, b in zip(g, w):
+            np.testing.assert_array_equal(a, b)
