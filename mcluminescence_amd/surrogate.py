@@ -125,16 +125,29 @@ def mc_curves(ctx: Context, theta: np.ndarray, n: int = 400, sims: int = 8,
     """Monte Carlo glow curves [N, 256]: recombinations per bin summed over the
     sims, divided by sims * N_e.  Parameter sets are uneven in work, so a chunk of
     several runs as one persistent launch (clusters taken largest expected work
-    first); one parameter set (equal clusters) runs one workgroup per cluster, with
-    no graph to build.  The counts are the same either way."""
+    first), summed over the sims on the device; one parameter set (equal clusters)
+    runs one workgroup per cluster.  The counts are the same either way."""
+    import torch
     theta = np.atleast_2d(theta)
     out = np.empty((theta.shape[0], N_BINS), np.float64)
     for c0 in range(0, theta.shape[0], chunk):
         part = theta[c0:c0 + chunk]
         runs = tl_table(part, n, sims)
-        flags = F_PERSISTENT if len(part) > 1 else 0
-        counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=flags)
-        out[c0:c0 + len(part)] = counts.reshape(len(part), sims, N_BINS).sum(axis=1) / (sims * n)
+        if len(part) == 1:
+            counts, _, _ = ctx.run_fixed_dt(runs, seed=seed + c0, n_bins=N_BINS, flags=0)
+            out[c0] = counts.sum(axis=0) / (sims * n)
+            continue
+        # counts stay on the device; only the sums over the sims (exact, int64) come back
+        plan = ctx.plan(runs, seed=seed + c0, n_bins=N_BINS, flags=F_PERSISTENT)
+        try:
+            counts = torch.empty((plan.info.total_clusters, N_BINS), dtype=torch.int32, device="cuda")
+            torch.cuda.current_stream().synchronize()
+            plan.launch(counts.data_ptr())
+            plan.sync()
+            sums = counts.view(len(part), sims, N_BINS).sum(dim=1, dtype=torch.int64).cpu().numpy()
+        finally:
+            plan.close()
+        out[c0:c0 + len(part)] = sums / (sims * n)
     return out
 
 
