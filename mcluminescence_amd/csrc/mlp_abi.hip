@@ -576,6 +576,8 @@ int mcl_de_create(void* mlp, const mcl_de_params* p, const float* target, void**
         return fail(m->ctx, MCL_EINVAL, "F %g must be in [0, 2] and CR %g in [0, 1]", p->F, p->CR);
     if (!std::isfinite(p->log_span) || !std::isfinite(p->out_scale) || !(p->inv_peak2 >= 0.0f))
         return fail(m->ctx, MCL_EINVAL, "log_span, out_scale and inv_peak2 must be finite");
+    if (p->flags & ~MCL_DE_LAYERED)
+        return fail(m->ctx, MCL_EINVAL, "flags 0x%x: unknown bits", (unsigned)p->flags);
     De* d = new (std::nothrow) De();
     if (!d) return fail(m->ctx, MCL_ENOMEM, "host allocation failed");
     d->m = m;

@@ -120,3 +120,18 @@ def test_invalid_parameters_are_rejected(net):
         with pytest.raises(MclError):
             F.NetworkDE(net, tgt, SCALE, fixed[:8], args["dim"], args["pop"], seed=1,
                         F=args["F"], CR=args["CR"])
+
+
+def test_unknown_flags_are_rejected(net):
+    import ctypes as C
+    import torch
+    from mcluminescence_amd._abi import DeParams
+    p = DeParams()
+    p.pop, p.dim, p.F, p.CR, p.seed = 64, 5, 0.5, 0.3, 1
+    p.fixed[0] = U_BETA
+    p.log_span, p.out_scale, p.inv_peak2 = 1.0, 1.0, 1.0
+    p.flags = 4
+    tgt = torch.ones(256, dtype=torch.float32, device="cuda")
+    h = C.c_void_p()
+    assert net.lib.mcl_de_create(net.handle, C.byref(p), tgt.data_ptr(), C.byref(h)) != 0
+    assert not h.value
