@@ -256,19 +256,19 @@ def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, bet
             if hist[-1] > hist[-2] * (1.0 - 1e-4):
                 break
         u, _ = de.best()
+        th = full_theta(u, beta)
+        judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
+        pred = S.predict_curves(net, th)[0]
+        err = float(S.nrmse(pred, judge)[0])
+        if log:
+            log(f"surrogate fit: {g} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
+        n_net = pop * (g + 1)
+        if err <= gate:
+            return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
+                             1, n_net, False, hist)
         xs, fs = de.population()        # the fallback's starting members
     finally:
         de.close()
-    th = full_theta(u, beta)
-    judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
-    pred = S.predict_curves(net, th)[0]
-    err = float(S.nrmse(pred, judge)[0])
-    if log:
-        log(f"surrogate fit: {g} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
-    n_net = pop * (g + 1)
-    if err <= gate:
-        return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
-                         1, n_net, False, hist)
     # fallback: Monte Carlo generations from the network's best candidates
     import torch
     order = torch.argsort(fs)[:fallback_pop]
