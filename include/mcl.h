@@ -17,7 +17,7 @@
  *   - A context owns one HIP stream and its device buffers; it is not
  *     thread-safe (use one context per host thread).
  *   - Clusters are ordered run-major then sim, the loop order of
- *     simulate.py:460-470 and the row order of its CSV (simulate.py:524-532).
+ *     simulate.py:36-46 and the row order of its CSV (simulate.py:100-110).
  */
 #ifndef MCL_H
 #define MCL_H
@@ -66,10 +66,10 @@ extern "C" {
  *               k_b [eV/K], D [Gy/s]                       (engine.py:48-60)
  *   exp_type_fp: rho_prime, holes, boundary_factor, N_e, e_ratio_start,
  *               T_start [degC], T_rate [K/s]               (tl_trap_lab.py:33-39,
- *                                                           simulate.py:464-468)
+ *                                                           simulate.py:40-44)
  * plus the fixed-dt schedule: dt [s] and n_steps.  Temperature at step n is
  * T_n = (T_start + 273.15) + T_rate * (n * dt), the start-of-step temperature
- * used by simulate.py:477-479.
+ * used by simulate.py:53-54.
  */
 typedef struct mcl_run_params {
     double alpha, b, s, E_cb, E_loc_1, E_loc_2, D0, Retrap, k_b, D;
@@ -135,7 +135,7 @@ int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out);
 
 /* ---- fixed-dt Monte Carlo -------------------------------------------------- */
 /*
- * Replaces the run x sim x step loop of simulate.py:460-516 (one event per step)
+ * Replaces the run x sim x step loop of simulate.py:36-92 (one event per step)
  * by the fixed-dt multi-recombination step (per live electron one Bernoulli
  * draw with p = 1 - exp(-k dt), k = s e^{-E_cb/kT} + b e^{-E_loc/kT - alpha r};
  * conflicts over a shared nearest hole go to the lowest electron index, the
@@ -148,7 +148,7 @@ int mcl_run_geometry(const mcl_run_params* run, mcl_geometry* out);
  *              row zero-padded after its own n_steps.
  *   live_steps [total_clusters] live electron-steps per cluster, or NULL.
  * The Dose filling path (D > 0) is not part of the fixed-dt kernel: runs with
- * D != 0 are rejected with MCL_EINVAL, matching simulate.py:466 which always
+ * D != 0 are rejected with MCL_EINVAL, matching simulate.py:42 which always
  * runs with dose_rate 0.
  */
 int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs,
@@ -293,6 +293,16 @@ int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_runs, const 
                    int32_t count_bytes, int64_t row_stride, const double* t_axis,
                    double bin_width, double win, uint32_t flags, double* curve,
                    int64_t curve_stride, int32_t* curve_len, mcl_stats* stats);
+
+/*
+ * Counts of one cluster of a sweep: cluster (run_index, sim) of the sweep that
+ * holds `run` at position run_index, identical to its row in mcl_run_fixed_dt,
+ * without simulating the sweep's other clusters.  For the one-cluster object
+ * TLTrapSim (tl_trap_lab.py:24-43), one simulation of one run.  Host pointers:
+ * counts [row_len] (n_bins, or run->n_steps for n_bins = 0), live_steps [1] or NULL.
+ */
+int mcl_run_cluster(void* ctx, const mcl_run_params* run, int32_t run_index, int32_t sim,
+                    uint64_t seed, int32_t n_bins, uint32_t* counts, uint64_t* live_steps);
 
 /*
  * State of cluster (run_index, sim) as the kernel holds it after max_steps

@@ -154,6 +154,8 @@ _SIGNATURES = {
     "mcl_sync": (C.c_int, [_P]),
     "mcl_cluster_state": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
+    "mcl_run_cluster": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
+                                  C.c_int32, _P, _P]),
     "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
     "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_uint32, C.c_uint64,
@@ -433,6 +435,18 @@ class Context:
                                         _ptr(e_live), _ptr(h_alive), _ptr(nn), _ptr(w))
         self.check(st, "mcl_cluster_state")
         return dict(e_xyz=e_xyz, h_xyz=h_xyz, e_live=e_live, h_alive=h_alive, nn=nn, w=w)
+
+    def run_cluster(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
+                    n_bins: int = 0):
+        """Counts [row_len] uint32 and live electron-steps of cluster (run_index, sim)
+        of a sweep holding `run` at run_index, simulating that cluster alone."""
+        row = int(n_bins) if n_bins > 0 else int(run.n_steps)
+        counts = np.zeros(row, np.uint32)
+        live = np.zeros(1, np.uint64)
+        st = self.lib.mcl_run_cluster(self.handle, C.byref(run), int(run_index), int(sim),
+                                      C.c_uint64(seed), int(n_bins), _ptr(counts), _ptr(live))
+        self.check(st, "mcl_run_cluster")
+        return counts, int(live[0])
 
     def plan(self, runs: Sequence[RunParams], seed: int = DEFAULT_SEED,
              n_bins: Optional[int] = None, flags: int = 0) -> "Plan":

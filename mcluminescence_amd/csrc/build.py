@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -44,17 +45,22 @@ def hipcc() -> str:
     return path
 
 
-def build(verbose: bool = False) -> Path:
-    objs = []
+def build(verbose: bool = False, jobs: int = 0) -> Path:
+    """Compiles every source (jobs at a time; 0: one per CPU, at most 8) and links."""
     tmp = PKG / "_objs"
     tmp.mkdir(exist_ok=True)
-    for src in SOURCES:
-        obj = tmp / (Path(src).stem + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(HERE / src), "-o", str(obj)]
+    objs = [str(tmp / (Path(src).stem + ".o")) for src in SOURCES]
+
+    def compile_one(k: int) -> None:
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(HERE / SOURCES[k]), "-o", objs[k]]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    jobs = jobs or min(8, os.cpu_count() or 1, len(SOURCES))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(compile_one, k) for k in range(len(SOURCES))]:
+            f.result()
     tmp_out = OUT.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp_out)]
     if verbose:

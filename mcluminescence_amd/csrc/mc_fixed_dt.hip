@@ -1,6 +1,6 @@
 // mc_fixed_dt.hip — fixed-dt trap/charge-kinetics Monte Carlo, one workgroup per cluster.
 //
-// Replaces the run x sim x step loop of simulate.py:460-516 (Box / Physics /
+// Replaces the run x sim x step loop of simulate.py:36-92 (Box / Physics /
 // TLTrapSim: engine.py:44-188, tl_trap_lab.py:24-60).  One workgroup of NW
 // wavefronts (1, 2, 4 or 8) owns one cluster; everything the step touches lives in LDS:
 //
@@ -42,7 +42,7 @@
 // holes of a cell range eight 16-byte records at a time, so one LDS latency
 // covers a chunk.
 //
-// Per step n (start-of-step temperature T_n = T0 + T_rate n dt, simulate.py:477-479):
+// Per step n (start-of-step temperature T_n = T0 + T_rate n dt, simulate.py:53-54):
 //   scalars   A = s e^{-E_cb/kT}, B_i = b e^{-E_loc_i/kT} in float64 (own exp),
 //             x_A = A dt, b_i dt in float32; the threshold of x_A once per step.
 //   fire      per live electron: u = Philox(n>>2, e, run, sim|FIRE)[n&3],
@@ -134,7 +134,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                           dist_evals);
     const float alpha_f = (float)R.alpha_nm;
 
-    // ---- step loop (simulate.py:475-516) -------------------------------------------
+    // ---- step loop (simulate.py:51-92) -------------------------------------------
     const int nb = a.n_bins > 0 ? a.n_bins : n_steps;   // n_bins == 0: one bin per step
     uint32_t* cnt = a.counts + (size_t)c * (size_t)a.row_len;
     const bool two_ch = R.two_channel != 0;

@@ -1184,6 +1184,53 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
     return st;
 }
 
+int mcl_run_cluster(void* ctx, const mcl_run_params* run, int32_t run_index, int32_t sim,
+                    uint64_t seed, int32_t n_bins, uint32_t* counts, uint64_t* live_steps)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!run || !counts) return fail(c, MCL_EINVAL, "null run or counts");
+    if (run_index < 0 || sim < 0) return fail(c, MCL_EINVAL, "run_index and sim must be >= 0");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    mcl_run_params one = *run;
+    one.sims = sim + 1;   // validated and sized as clusters 0..sim; only `sim` runs
+    Plan* p = nullptr;
+    int st = make_plan(c, &one, 1, &run_index, seed, n_bins, 0, &p);
+    if (st) return st;
+    const size_t row = (size_t)p->row_len;
+    uint32_t* d_counts = nullptr;
+    uint64_t* d_live = nullptr;
+    const int32_t zero = 0;
+    hipError_t e = hipMemcpyAsync(p->d_crun, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->d_csim, &sim, sizeof sim, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);   // host values leave scope
+    p->total = 1;
+    if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&d_counts, sizeof(uint32_t) * row);
+    if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&d_live, sizeof(uint64_t));
+    if (e != hipSuccess) {
+        mcl::pool_release(c, d_counts, true);
+        plan_free(p, true);
+        return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "run_cluster setup: %s",
+                    hipGetErrorString(e));
+    }
+    st = plan_launch(p, d_counts, d_live, nullptr);
+    uint64_t live = 0;
+    if (st == MCL_OK) {
+        e = hipMemcpyAsync(counts, d_counts, sizeof(uint32_t) * row, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&live, d_live, sizeof live, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = fail(c, MCL_EDEVICE, "run_cluster: %s", hipGetErrorString(e));
+    }
+    if (st == MCL_OK && live_steps) *live_steps = live;
+    if (e != hipSuccess || st != MCL_OK) (void)hipDeviceSynchronize();
+    mcl::pool_release(c, d_counts, true);
+    mcl::pool_release(c, d_live, true);
+    plan_free(p, true);
+    return st;
+}
+
 int mcl_cluster_state(void* ctx, const mcl_run_params* run, int32_t run_index, int32_t sim,
                       uint64_t seed, int32_t max_steps, float* e_xyz, float* h_xyz,
                       uint8_t* e_live, uint8_t* h_alive, uint32_t* nn, float* w)

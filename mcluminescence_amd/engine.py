@@ -83,7 +83,7 @@ def physics_of(run_cfg: Config) -> Physics:
 
 
 def schedule(run_cfg: Config) -> Dict[str, float]:
-    """Fixed-dt schedule of one run (simulate.py:464-469).
+    """Fixed-dt schedule of one run (simulate.py:40-45).
 
     TL: dt = max_dt / T_rate, the reference's temperature-step cap; isothermal:
     dt = exp_type_fp.dt if given, else max_dt.  n_steps covers the duration."""

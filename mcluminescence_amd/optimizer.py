@@ -8,7 +8,7 @@ src/class/optimizer.py (DE over ten parameters against a lab objective).
 Same bounds, parameter order, DE settings (randtobest1bin, sobol init, mutation
 0.5, recombination 0.3, tol 1e-7, polish) and result file
 (results/lab_sims/result_{exp}.csv, one row param_0..param_9, mse appended per
-run) as optimizer.py:32-147.  The objective is TL_lab or ISO_lab on the lab
+run) as optimizer.py:32-136.  The objective is TL_lab or ISO_lab on the lab
 kernel; the whole DE population is evaluated in one kernel launch (scipy's
 vectorized objective: candidates x lab rows as one batch of clusters) instead of
 one process per candidate.  ``sims`` clusters per lab row average the simulated
@@ -112,9 +112,12 @@ def run_one_sim(cfg: Config, exp: str, plot: bool = False, *, ctx: Optional[Cont
 
 def train(cfg: Config, exp: str, *, gens: int = 7, pop: int = 15, sims: int = 1,
           ctx: Optional[Context] = None, table=None, seed: Optional[int] = None,
-          polish: bool = True, csv_path: Optional[Path] = None, disp: bool = True):
-    """DE as optimizer.py:101-116, with the population batched on the GPU; appends
-    the best row to results/lab_sims/result_{exp}.csv."""
+          polish: bool = True, csv_path: Optional[Path] = None, disp: bool = True,
+          replot: bool = True):
+    """DE as optimizer.py:101-121, with the population batched on the GPU: appends
+    the best row to results/lab_sims/result_{exp}.csv, then runs the best
+    parameters once more with the lab plot written (replot=False skips the plot,
+    not the run).  The result gains ``best_rerun_mse``, that run's objective."""
     from scipy.optimize import differential_evolution
     import pandas as pd
     if ctx is None:
@@ -141,6 +144,9 @@ def train(cfg: Config, exp: str, *, gens: int = 7, pop: int = 15, sims: int = 1,
     pd.DataFrame([list(result.x) + [float(result.fun)]], columns=cols).to_csv(
         csv_path, mode="a", index=False, header=not csv_path.exists())
     print(f"Saved run to {csv_path}")
+    # quick check of the best parameters, with the lab plot (optimizer.py:119-121)
+    result.best_rerun_mse = run_one_sim(cfg_with_params(cfg, result.x), exp, plot=replot, ctx=ctx,
+                                        sims=sims, table=tab)
     return result
 
 

@@ -5,7 +5,7 @@
 
 Same registry keys (TL, TL_norm, iso, bg), plotter signature
 ``(cfg, ax, x_ax, lum, e_ratio, configs)`` and output files
-results/plots/{key}_run_{tag}.png (plots.py:27-37,185-220).  The sweep comes
+results/plots/{key}_run_{tag}.png (plots.py:28-37,138-171).  The sweep comes
 from ``simulate(cfg)``, or with ``from_csv`` from the results the driver wrote:
 exp_{tag}.csv plus its time-axis side-car exp_{tag}_x_ax.csv.
 
@@ -13,7 +13,7 @@ The glow curves of TL / TL_norm (plots.py:39-66: the first sim's time axis, the
 mean over sims of Lum, 1-degree bins, a 50-degree boxcar) are computed on the
 GPU by mcl_glow_curve for every run in one launch; matplotlib only draws.
 plot_iso takes the configs dict as its last argument (the reference's registry
-passes it, plots.py:210, while its plot_iso signature omits it and indexes the
+passes it, plots.py:167, while its plot_iso signature omits it and indexes the
 root config, plots.py:103-108).
 """
 from __future__ import annotations
@@ -161,7 +161,7 @@ PLOTTERS: Dict[str, Callable] = {
 def plot_all(cfg: Config, *, from_csv: bool = False, ctx: Optional[Context] = None,
              results_dir: Optional[Path] = None) -> List[Path]:
     """Simulate (or read the saved sweep) and draw every requested plotter
-    (plots.py:188-216); returns the files written."""
+    (plots.py:149-171); returns the files written."""
     import matplotlib
     matplotlib.use("Agg")
     import matplotlib.pyplot as plt

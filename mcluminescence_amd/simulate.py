@@ -22,7 +22,7 @@ two modes (config key ``mode``, default ``fixed_dt``):
             ``Lum`` is the number of recombinations in each step, ``x_ax`` the end
             time of each step, ``electron_ratio`` the trapped fraction after it.  The
             step is the reference's temperature-step cap ``max_dt / T_rate`` for
-            ramps (simulate.py:468-469) and ``dt`` (or ``max_dt``) for isothermal runs.
+            ramps (simulate.py:44-45) and ``dt`` (or ``max_dt``) for isothermal runs.
   event     the reference's own loop, one event per step (``Lum`` in {0, 1}, the
             step cap ``max_dt / T_rate``, ``x_ax`` the event times), on the event
             kernel.
@@ -90,14 +90,14 @@ def run_sweep(configs: Dict[int, Config], steps: int, sims: int, *, ctx: Optiona
 
 def run_sweep_event(configs: Dict[int, Config], steps: int, sims: int, *,
                     ctx: Optional[Context] = None, seed: int = DEFAULT_SEED):
-    """The reference's event loop for every run x sim (simulate.py:460-516), one
+    """The reference's event loop for every run x sim (simulate.py:36-92), one
     kernel launch; returns x_ax, Lum, electron_ratio ([steps, sims, n_runs])."""
     ctx = ctx or default_context()
     n_runs = len(configs)
     params, durations = [], []
     for r in range(n_runs):
         mc = configs[r].exp_type_fp
-        max_dt = float(mc.get("max_dt", 0.5))                 # simulate.py:468
+        max_dt = float(mc.get("max_dt", 0.5))                 # simulate.py:44
         params.append(run_params(configs[r], sims=sims, n_steps=1, dt=max_dt))
         durations.append(float(mc.get("duration", 0.0)))
     try:
@@ -114,7 +114,7 @@ def run_sweep_event(configs: Dict[int, Config], steps: int, sims: int, *,
 
 
 def results_frame(lum: np.ndarray, e_ratio: np.ndarray):
-    """The reference's CSV layout (simulate.py:519-532): run-major, then sim, then step."""
+    """The reference's CSV layout (simulate.py:95-110): run-major, then sim, then step."""
     import pandas as pd
     steps, sims, runs = lum.shape
     return pd.DataFrame({

@@ -184,8 +184,14 @@ class Network:
         self.set_params(default_params(self.desc, seed) if params is None else params)
 
     def close(self) -> None:
+        """Free the network after the work queued on its own stream: the destroy
+        synchronises the stream the context is set to, so it runs on that stream."""
         if getattr(self, "handle", None):
-            self.lib.mcl_mlp_destroy(self.handle)
+            if self._torch_stream is not None:
+                with self._on_stream():
+                    self.lib.mcl_mlp_destroy(self.handle)
+            else:
+                self.lib.mcl_mlp_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

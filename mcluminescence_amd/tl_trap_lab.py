@@ -3,7 +3,7 @@
 
 The reference's object is driven one event at a time from Python
 (``_update_lifetimes`` / ``_filling_time`` / ``box.remove_pair``, called by
-simulate.py:475-516).  Here a cluster's whole step loop is one kernel launch, so
+simulate.py:51-92).  Here a cluster's whole step loop is one kernel launch, so
 the object keeps the constructor and the attributes callers read
 (``cfg``, ``mc``, ``phys``, ``box.electrons``, ``box.holes``, ``box.nearest()``)
 and adds ``run()``; the per-event private methods have no equivalent.
@@ -109,12 +109,11 @@ class TLTrapSim:
         return self._ctx
 
     def run(self, n_steps: Optional[int] = None, dt: Optional[float] = None) -> np.ndarray:
-        """Per-step recombination counts of this cluster over its schedule."""
-        p = run_params(self.cfg, sims=self.sim + 1, e_ratio_start=self.e_ratio_start,
-                       n_steps=n_steps, dt=dt)
-        runs = [p] * (self.run_index + 1)
-        counts, _, _ = self._context().run_fixed_dt(runs, seed=self.seed, n_bins=0)
-        return counts[self.run_index * (self.sim + 1) + self.sim]
+        """Per-step recombination counts of this cluster over its schedule (the
+        row (run_index, sim) of the sweep's counts; only this cluster is simulated)."""
+        p = run_params(self.cfg, sims=1, e_ratio_start=self.e_ratio_start, n_steps=n_steps, dt=dt)
+        counts, _ = self._context().run_cluster(p, self.run_index, self.sim, seed=self.seed)
+        return counts
 
     def TL_lab(self, csv_name: str, *, plot: bool = False, **kw) -> float:
         """MSE against a TL lab table (tl_trap_lab.py:65-118); keywords of lab.tl_lab."""

@@ -64,6 +64,21 @@ def tunnelling_tl12(sims: int = 2, n: int = 2000, T_rate: float = 1.0) -> List[R
                            T_rate=T_rate, dt=dt, n_steps=n_steps, sims=sims)]
 
 
+def tunnelling_iso(rho_prime: float, *, n: int = 2000, fill: float = 0.05, dt: float = 1.0,
+                   n_steps: int = 4000, sims: int = 256, s_eff: float = 1.0, T_K: float = 500.0,
+                   E_loc: float = 1.0, alpha: float = 4e9) -> List[RunParams]:
+    """Tunnelling-only isothermal decay (the regime of Huntley's closed form,
+    functions.py:264-268): the band is off (E_cb = 1000 eV, as in basicTL12), one
+    localisation level E_loc at T_K with b chosen so that the effective tunnelling
+    frequency b e^{-E_loc / kT} is s_eff, n core holes (N_e = n) filled to fill, so
+    electrons are few against the holes."""
+    b = s_eff / math.exp(-E_loc / (K_B * T_K))
+    return [RunParams.make(alpha=alpha, b=b, s=1e12, E_cb=1000.0, E_loc_1=E_loc, E_loc_2=E_loc,
+                           D0=D0, Retrap=0.5, k_b=K_B, D=0.0, rho_prime=rho_prime,
+                           holes=float(n), boundary_factor=1.2, N_e=float(n), e_ratio_start=fill,
+                           T_start=T_K - 273.15, T_rate=0.0, dt=dt, n_steps=n_steps, sims=sims)]
+
+
 def uneven_sweep(n_sets: int = 10000, seed: int = 2024, n_bins: int = 256,
                  n_steps: int = 1024) -> List[RunParams]:
     """Batched parameter sweep with uneven clusters: half TL (0.5-5 K/s, 300-750 K),

@@ -11,7 +11,7 @@ electron's wait redrawn every step):
   Box._rebuild / seed / add_electron / remove_pair  src/class/engine.py:113-175
   TLTrapSim.__init__ / _update_lifetimes /
       _filling_time .............................. src/class/tl_trap_lab.py:27-60
-  simulate() run x sim x step loop ............... src/class/simulate.py:460-516
+  simulate() run x sim x step loop ............... src/class/simulate.py:36-92
 
 including its quirks: dose filling reads ``phys["dose_rate"]`` (never set by the
 shipped YAML, so lambda = 1e-20 and one Exp(1e20) draw per step); an empty
@@ -150,10 +150,10 @@ class Sim:
 
 def simulate_run(exp: Dict, phys: Dict, sims: int, steps: int, seed: int,
                  first_sim: int = 0):
-    """simulate.py:460-516 for one run: returns x_ax, lum, e_ratio of shape (steps, sims).
+    """simulate.py:36-92 for one run: returns x_ax, lum, e_ratio of shape (steps, sims).
 
     Raises IndexError, like the reference, if the loop needs more than ``steps``
-    steps before ``duration`` is reached (simulate.py:488)."""
+    steps before ``duration`` is reached (simulate.py:64)."""
     x_ax = np.zeros((steps, sims))
     lum = np.zeros((steps, sims))
     e_ratio = np.zeros((steps, sims))

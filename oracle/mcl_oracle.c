@@ -11,7 +11,7 @@
  *   nearest hole   engine.py:113-119 (argmin, first index on ties), 154-175
  *   rates          engine.py:65-77  (k = s e^{-E_cb/kT} + b e^{-E_loc/kT - alpha r},
  *                                    E_loc = E_loc_2 if U < Retrap else E_loc_1)
- *   step loop      simulate.py:470-516 (start-of-step temperature)
+ *   step loop      simulate.py:46-92 (start-of-step temperature)
  *   conflicts      src/est_params/functions.py:228-239 (first electron keeps the hole);
  *                  a loser re-searches its nearest hole and retries within the same
  *                  step, accepted with probability T_new / T_old (thinning), up to
@@ -426,7 +426,7 @@ static uint64_t run_cluster(const mcl_run_params* p, int run, int sim, const uin
     const int step_limit = (fs && fs->max_steps < n_steps) ? fs->max_steps : n_steps;
     for (int n = 0; n < step_limit && c.n_live > 0; ++n) {
         live_steps += (uint64_t)c.n_live;
-        /* per-step float64 scalars (simulate.py:477-479, engine.py:65-73) */
+        /* per-step float64 scalars (simulate.py:53-54, engine.py:65-73) */
         const double t = (double)n * p->dt;
         const double T = T0_K + p->T_rate * t;
         const double kT = p->k_b * T;

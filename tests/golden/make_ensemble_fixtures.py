@@ -1,5 +1,5 @@
 """Ensemble fixtures of the event-mode oracle (oracle/event_mc.py, the numpy
-restatement of simulate.py:460-516) for the GPU ensemble test.
+restatement of simulate.py:36-92) for the GPU ensemble test.
 
 For each case the oracle runs `sims` independent clusters in event mode; events
 are binned by time onto the fixed-dt step grid (oracle.event_mc.bin_events) and

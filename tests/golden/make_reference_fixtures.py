@@ -6,7 +6,9 @@ Reads (in this container only) /root/reference/results/simulations/exp_*.csv and
     simulate.py result files (TL12 + basicTL12, 4 heating rates x 2 sims),
   - the CSV schema of the current writer (header, dtypes, first and last rows,
     row count) from exp_TLwithDose_emptystart.csv,
-  - the processed lab tables the objectives read.
+  - the processed lab tables the objectives read,
+  - the parameter rows and recorded MSEs of the reference's lab fits
+    (results/lab_sims/result_*.csv), as written (raw field strings).
 Run: python tests/golden/make_reference_fixtures.py
 """
 import json
@@ -57,6 +59,12 @@ def main():
     for f in sorted((REF / "data" / "processed").glob("*.csv")):
         lab[f.stem] = pd.read_csv(f).to_dict(orient="list")
     fx["lab_data"] = lab
+    fits = {}
+    for f in sorted((REF / "results" / "lab_sims").glob("result_*.csv")):
+        lines = f.read_text().splitlines()
+        fits[f.stem] = {"header": lines[0].split(","),
+                        "rows": {str(i + 1): ln.split(",") for i, ln in enumerate(lines) if i > 0}}
+    fx["lab_fit_results"] = fits
     OUT.write_text(json.dumps(fx, indent=1))
     print("wrote", OUT)
 

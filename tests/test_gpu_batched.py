@@ -99,3 +99,32 @@ def test_reused_device_buffers_give_fresh_context_results(gpu_ctx):
     for g, w in zip(got, want):
         for a, b in zip(g, w):
             np.testing.assert_array_equal(a, b)
+
+
+def test_full_uneven_sweep_graph_launch_matches_replay(gpu_ctx):
+    """The bench's sweep at its quoted size: 10,000 sets, 64-2048 electrons, 1024
+    steps, 256 bins, one persistent graph launch with the tuning table's knobs,
+    counts and live electron-steps identical to the replay harness."""
+    import torch
+    runs = W.uneven_sweep()
+    knobs = autotune.apply(gpu_ctx, runs)
+    print(f"uneven sweep, full size: {len(runs)} sets, 1024 steps, 256 bins, knobs {knobs}",
+          flush=True)
+    try:
+        plan = gpu_ctx.plan(runs, n_bins=256, flags=F_PERSISTENT | F_GRAPH)
+        n, row = plan.info.total_clusters, plan.info.row_len
+        counts = torch.zeros((n, row), dtype=torch.int32, device="cuda")
+        live = torch.zeros(n, dtype=torch.int64, device="cuda")
+        plan.launch(counts.data_ptr(), live.data_ptr())
+        plan.sync()
+        got = counts.cpu().numpy().view(np.uint32)
+        got_live = live.cpu().numpy().view(np.uint64)
+        plan.close()
+    finally:
+        autotune.apply(gpu_ctx, [], {})
+    print("replay harness on the same 10,000 sets", flush=True)
+    ref, ref_live = orc.run_fixed_dt(runs, n_bins=256, threads=16)
+    bad = np.nonzero((got != ref).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} sets differ, first {bad[:5]}"
+    np.testing.assert_array_equal(got_live, ref_live)
+    assert int(ref.sum()) > 0

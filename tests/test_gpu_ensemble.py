@@ -2,7 +2,7 @@
 algorithm: per-bin mean recombinations per cluster against the event-mode
 oracle's ensembles (tests/golden/ensemble_event_mode.json, made by
 tests/golden/make_ensemble_fixtures.py from oracle/event_mc.py, the numpy
-restatement of simulate.py:460-516), within 3 standard errors per bin; glow peak
+restatement of simulate.py:36-92), within 3 standard errors per bin; glow peak
 temperatures within 1 K of the oracle's and of the Randall-Wilkins closed form.
 
 Statistics: z = (mean_gpu - mean_oracle) / sqrt(v_o / S_o + v_g / S_g) per bin,

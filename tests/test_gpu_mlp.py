@@ -167,3 +167,24 @@ def test_invalid_batches_are_rejected(gpu_ctx):
         net.grad(x, t)                    # not a multiple of 64
     with pytest.raises(ValueError):
         net.forward(torch.zeros((200, 6), device="cuda"))   # beyond max_batch
+
+
+def test_close_right_after_unsynchronised_training(gpu_ctx):
+    # the destroy waits for the network's own stream, where its steps were queued:
+    # freeing the weights under running kernels would fault or corrupt later work
+    import torch
+    print("close after queued AdamW steps, then a fresh network's forward", flush=True)
+    rng = np.random.default_rng(12)
+    x = torch.from_numpy(rng.uniform(-1, 1, (4096, 6)).astype(np.float32)).cuda()
+    t = torch.from_numpy(rng.uniform(0, 1, (4096, 256)).astype(np.float32)).cuda()
+    idx = torch.arange(4096, dtype=torch.int32, device="cuda")
+    for k in range(3):
+        net = _net(gpu_ctx, max_batch=4096, seed=20 + k)
+        for step in range(1, 6):
+            net.train_step(x, t, idx, AdamW(1e-3, 0.9, 0.999, 1e-8, 0.0, step))
+        net.close()                       # no synchronisation before it
+    fresh = _net(gpu_ctx, max_batch=4096, seed=30)
+    ref = _net(gpu_ctx, max_batch=4096, seed=30)
+    np.testing.assert_array_equal(fresh.forward(x).cpu().numpy(), ref.forward(x).cpu().numpy())
+    fresh.close()
+    ref.close()

@@ -28,13 +28,17 @@ def test_batched_objective_matches_per_candidate_scale(gpu_ctx):
     assert abs(one - f[0]) < 0.02
 
 
-def test_train_and_replay(gpu_ctx, tmp_path):
+def test_train_and_replay(gpu_ctx, tmp_path, monkeypatch):
+    monkeypatch.setenv("MCL_PROJECT_ROOT", str(tmp_path))      # the re-run's plot goes here
     cfg = config.compose(["exp_type_fp=TLlab", "physics_fp=lab_TL"])
     print("DE on TL_lab: 2 generations x 40 candidates, batched", flush=True)
     res = optimizer.train(cfg, "tl_clbr", gens=2, pop=4, sims=4, ctx=gpu_ctx,
                           table=LAB["CLBR_IRSL50_0.25KperGy"], seed=1, polish=False,
                           csv_path=tmp_path / "result_tl_clbr.csv", disp=False)
     assert np.isfinite(res.fun) and res.x.shape == (10,)
+    # the best parameters were run again with the plot written (optimizer.py:119-121)
+    assert np.isfinite(res.best_rerun_mse)
+    assert (tmp_path / "results" / "lab_sims" / "CLBR_IRSL50_0.25KperGy.png").exists()
     for (lo, hi), v in zip(optimizer.DEFAULT_BOUNDS, res.x):
         assert lo <= v <= hi
     df = pd.read_csv(tmp_path / "result_tl_clbr.csv")

@@ -137,3 +137,17 @@ def test_grid_edge_cases_bit_exact(gpu_ctx):
         np.testing.assert_array_equal(st["e_live"], ref["e_live"])
         live = st["e_live"].astype(bool)
         np.testing.assert_array_equal(st["nn"][live], ref["nn"][live])
+
+
+@pytest.mark.parametrize("T_K", [540.0, 560.0])
+def test_conflict_heavy_iso_4096_closed_form(gpu_ctx, T_K):
+    # the full decay workload where conflicts bind (p = 0.35 / 0.72): bit-exact
+    # against the replay, and band-only survival N0 (1 - p)^n with mean life 1 / p
+    from test_oracle_kinetics import _band_decay_check
+    import math
+    runs = W.iso_decay_4096x400()
+    runs[0].T_start = T_K - 273.15
+    g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+    _assert_same(g, gl, o, ol)
+    p = -math.expm1(-1e13 * math.exp(-1.4 / (8.617343e-5 * T_K)) * 0.5)
+    _band_decay_check(g, gl, 400 * 4096, p)

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from mcluminescence_amd import config, engine
+from mcluminescence_amd._abi import RunParams
 from mcluminescence_amd.simulate import simulate
 from mcluminescence_amd.tl_trap_lab import TLTrapSim
 from oracle import orc
@@ -72,3 +73,14 @@ def test_tltrapsim_box_view(gpu_ctx):
     gone = int(counts[:200].sum())
     assert sim.box.electrons.shape[0] == 500 - gone
     assert sim.box.holes.shape[0] == 3455 - gone
+
+
+def test_tltrapsim_run_is_the_sweeps_row(gpu_ctx):
+    # cluster (run_index 2, sim 5) alone gives exactly its row of the full sweep
+    run = engine.initialize_runs(config.compose())[0]
+    sim = TLTrapSim(run, e_ratio_start=0.25, ctx=gpu_ctx, run_index=2, sim=5)
+    counts = sim.run(n_steps=300)
+    p = RunParams.make(**{**sim._params.as_dict(), "sims": 6, "n_steps": 300})
+    full, _, _ = gpu_ctx.run_fixed_dt([p, p, p], n_bins=0)
+    np.testing.assert_array_equal(counts, full[2 * 6 + 5])
+    assert counts.sum() > 0

@@ -26,22 +26,43 @@ GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_outputs.json"
 MAX_ROUNDS = 12
 
 
+def _band_decay_check(counts, live, N0, p):
+    """Band-only survival N0 (1 - p)^n at every step with survivors expected, and
+    the mean life 1 / p, within 4 standard deviations."""
+    gone = np.cumsum(counts.sum(axis=0))
+    n = 1
+    while N0 * (1 - p) ** n >= 10:
+        surv = N0 - gone[n - 1]
+        expect = N0 * (1 - p) ** n
+        sd = math.sqrt(N0 * (1 - p) ** n * (1 - (1 - p) ** n))
+        assert abs(surv - expect) < 4 * sd + 1, (n, surv, expect, sd)
+        n += 1 if n < 10 else max(1, n // 4)
+    mean_life = live.sum() / N0                    # E[steps alive] = 1/p
+    sd = math.sqrt((1 - p) / p ** 2 / N0)
+    assert abs(mean_life - 1 / p) < 4 * sd, (mean_life, 1 / p, sd)
+
+
 def test_iso_band_decay_matches_closed_form():
     r = W.iso_decay_4096x400(sims=1024)[0]
     counts, live = orc.run_fixed_dt([r], threads=8)
     k = 1e13 * math.exp(-1.4 / (8.617343e-5 * 500.0))
     p = -math.expm1(-k * 0.5)
     assert k == pytest.approx(0.07738, rel=1e-3) and p == pytest.approx(0.03795, rel=1e-3)
-    N0 = 400 * 1024
-    gone = np.cumsum(counts.sum(axis=0))
-    for n in (1, 5, 10, 26, 52, 100):
-        surv = N0 - gone[n - 1]
-        expect = N0 * (1 - p) ** n
-        sd = math.sqrt(N0 * (1 - p) ** n * (1 - (1 - p) ** n))
-        assert abs(surv - expect) < 4 * sd + 1, (n, surv, expect, sd)
-    mean_life = live.sum() / N0                    # E[steps alive] = 1/p
-    sd = math.sqrt((1 - p) / p ** 2 / N0)
-    assert abs(mean_life - 1 / p) < 4 * sd
+    _band_decay_check(counts, live, 400 * 1024, p)
+
+
+@pytest.mark.parametrize("T_K,p_expect", [(540.0, 0.349), (560.0, 0.716)])
+def test_iso_band_decay_closed_form_where_conflicts_bind(T_K, p_expect):
+    # at p = 0.35 / 0.71 a third to two thirds of the electrons fire each step and
+    # many share a nearest hole: the survival is exact only if every conflict loser
+    # gets its same-step retry (the T_new / T_old thinning on its new hole) and the
+    # MCL_MAX_ROUNDS cap never binds
+    r = W.iso_decay_4096x400(sims=1024)[0]
+    r.T_start = T_K - 273.15
+    counts, live = orc.run_fixed_dt([r], threads=8)
+    p = -math.expm1(-1e13 * math.exp(-1.4 / (8.617343e-5 * T_K)) * 0.5)
+    assert p == pytest.approx(p_expect, abs=0.001)
+    _band_decay_check(counts, live, 400 * 1024, p)
 
 
 def test_tl_band_peak_within_1K_of_randall_wilkins():
