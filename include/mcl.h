@@ -305,6 +305,22 @@ int mcl_run_cluster(void* ctx, const mcl_run_params* run, int32_t run_index, int
                     uint64_t seed, int32_t n_bins, uint32_t* counts, uint64_t* live_steps);
 
 /*
+ * Per-set curves from per-step counts (device pointers, asynchronous on the
+ * context's stream): for set p and bin j,
+ *   out[p][j] = sum over its sims k < sims and steps n in [edges[j], edges[j+1])
+ *               of counts[p * sims + k][n]
+ * with counts rows of row_stride u32 (mcl_run_fixed_dt / a plan with n_bins = 0)
+ * and edges a host array of n_bins + 1 increasing step indices from >= 0, the
+ * last <= row_stride and <= 16384.  The surrogate's isothermal curves use
+ * log-spaced edges.  No counterpart in the reference (its binning is by
+ * temperature, plots.py:39-47; mcl_glow_curve).  MCL_ERANGE beyond MCL_BIN_MAX
+ * bins or 16384 steps.
+ */
+#define MCL_BIN_MAX 512
+int mcl_bin_counts(void* ctx, const uint32_t* counts_dev, int64_t n_sets, int32_t sims,
+                   int64_t row_stride, const int32_t* edges, int32_t n_bins, uint64_t* out_dev);
+
+/*
  * State of cluster (run_index, sim) as the kernel holds it after max_steps
  * steps (0 = the initial state): coordinates [nm] in float32, which electrons
  * and holes are left, the incrementally maintained nearest live hole per

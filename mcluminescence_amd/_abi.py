@@ -156,6 +156,7 @@ _SIGNATURES = {
                                     C.c_int32, _P, _P, _P, _P, _P, _P]),
     "mcl_run_cluster": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
                                   C.c_int32, _P, _P]),
+    "mcl_bin_counts": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_int64, _P, C.c_int32, _P]),
     "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
     "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_uint32, C.c_uint64,
@@ -262,6 +263,10 @@ class Context:
         if st != MCL_OK:
             raise MclError(st, f"mcl_create(device={device}) failed: no usable HIP device")
         self.handle = h
+
+    def sync(self) -> None:
+        """Wait for the work queued on the context's stream."""
+        self.check(self.lib.mcl_sync(self.handle), "mcl_sync")
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -435,6 +440,15 @@ class Context:
                                         _ptr(e_live), _ptr(h_alive), _ptr(nn), _ptr(w))
         self.check(st, "mcl_cluster_state")
         return dict(e_xyz=e_xyz, h_xyz=h_xyz, e_live=e_live, h_alive=h_alive, nn=nn, w=w)
+
+    def bin_counts(self, counts_ptr: int, n_sets: int, sims: int, row_stride: int,
+                   edges: Sequence[int], out_ptr: int) -> None:
+        """mcl_bin_counts on device buffers (asynchronous on the context's stream):
+        out[p][j] = sum of counts[p*sims+k][edges[j]:edges[j+1]] over the set's sims."""
+        e = np.ascontiguousarray(np.asarray(edges, np.int32))
+        st = self.lib.mcl_bin_counts(self.handle, C.c_void_p(counts_ptr), int(n_sets), int(sims),
+                                     int(row_stride), _ptr(e), len(e) - 1, C.c_void_p(out_ptr))
+        self.check(st, "mcl_bin_counts")
 
     def run_cluster(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
                     n_bins: int = 0):

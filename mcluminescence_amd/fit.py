@@ -14,10 +14,15 @@ Two ways to evaluate a candidate population:
                  falls back to Monte Carlo generations seeded with the network's
                  final population.
 
-The unknowns are E, s, alpha, rho and the fill fraction; the heating rate is the
-experiment's.  The curve depends on alpha and rho only through
-rho' = rho / (3/(4 pi) alpha^3) (tl_trap_lab.py:33), so recovery is reported for
-E, s, rho' and fill.
+The unknowns are E, s, alpha, rho and the fill fraction; the sixth input is the
+experiment's own: the heating rate of a TL curve, the temperature of an
+isothermal decay (the kind of the curve, surrogate.TL or surrogate.ISO, the
+reference's TL_lab and ISO_lab objectives, optimizer.py:73-78).  A TL curve
+depends on alpha and rho only through rho' = rho / (3/(4 pi) alpha^3)
+(tl_trap_lab.py:33), so TL recovery is reported for E, s, rho' and fill.  An
+isothermal decay at one temperature depends on E and s only through the rate
+k = s e^{-E/kT} (the tunnelling term is below 1e-4 of it over the box), so its
+recovery is reported for log10 k and fill.
 """
 from __future__ import annotations
 
@@ -35,23 +40,32 @@ from .workloads import rho_prime_for
 
 N_FIT = 5          # E, s, alpha, rho, fill
 JUDGE_SIMS, JUDGE_N = 64, 1000
+K_B = 8.617343e-5
 
 
-def full_theta(u5: np.ndarray, beta: float) -> np.ndarray:
-    """Encoded fit vector(s) [.., 5] -> physical parameters [.., 6] with the fixed beta."""
+def full_theta(u5: np.ndarray, fixed: float, kind: S.CurveKind = S.TL) -> np.ndarray:
+    """Encoded fit vector(s) [.., 5] -> physical parameters [.., 6] with the fixed
+    sixth input (heating rate for TL, temperature for ISO)."""
     u5 = np.atleast_2d(u5)
-    th = S.decode(np.concatenate([u5, np.zeros((u5.shape[0], 1))], axis=1))
-    th[:, 5] = beta
+    th = kind.decode(np.concatenate([u5, np.zeros((u5.shape[0], 1))], axis=1))
+    th[:, 5] = fixed
     return th
 
 
+def iso_rate(theta: np.ndarray) -> np.ndarray:
+    """k = s e^{-E / kT_iso} [1/s] of isothermal parameter rows."""
+    th = np.atleast_2d(theta)
+    return th[:, 1] * np.exp(-th[:, 0] / (K_B * th[:, 5]))
+
+
 def lab_curve(ctx: Context, theta: np.ndarray, peak_counts: float = 1e4,
-              rng: Optional[np.random.Generator] = None, seed: int = DEFAULT_SEED + 101):
+              rng: Optional[np.random.Generator] = None, seed: int = DEFAULT_SEED + 101,
+              kind: S.CurveKind = S.TL):
     """Synthetic measured curve: Monte Carlo at theta (64 clusters x 1000 electrons)
     scaled to a peak of peak_counts, with Poisson counting noise.  Returns (counts,
     scale) where scale converts a fraction-of-N_e curve into counts."""
     rng = rng or np.random.default_rng(0)
-    c = S.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)[0]
+    c = kind.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)[0]
     scale = peak_counts / max(c.max(), 1e-30)
     return rng.poisson(c * scale).astype(np.float64), scale
 
@@ -123,7 +137,7 @@ class NetworkDE:
 
     def __init__(self, net: S.Network, y: np.ndarray, scale: float, fixed: Sequence[float],
                  dim: int, pop: int, seed: int, F: float = 0.5, CR: float = 0.3,
-                 layered: bool = False):
+                 layered: bool = False, peak: Optional[float] = None):
         import torch
         from ._abi import DE_LAYERED, DeParams
         self.net, self.dim, self.pop = net, dim, pop
@@ -134,7 +148,7 @@ class NetworkDE:
             p.fixed[k] = float(v)
         p.log_span = math.log1p(1.0 / net.c0)
         p.out_scale = net.c0 * scale
-        p.inv_peak2 = 1.0 / max(float(np.max(y)), 1.0) ** 2
+        p.inv_peak2 = 1.0 / max(float(np.max(y)) if peak is None else float(peak), 1.0) ** 2
         p.flags = DE_LAYERED if layered else 0
         h = C.c_void_p()
         with net._on_stream(self.target):
@@ -181,7 +195,7 @@ class NetworkDE:
 
 @dataclass
 class FitResult:
-    theta: np.ndarray                 # physical parameters [6] (beta included)
+    theta: np.ndarray                 # physical parameters [6] (beta / T_iso included)
     objective_mc: float               # objective of theta on the Monte Carlo judge curve
     judge_nrmse: float                # network vs judge curve at theta (nan for fit_mc)
     seconds: float
@@ -189,65 +203,66 @@ class FitResult:
     net_candidates: int               # candidate curves evaluated by the network
     fallback: bool = False
     history: List[float] = field(default_factory=list)
+    kind: str = "tl"
 
     def recovery(self, truth: np.ndarray) -> dict:
-        """Errors against the true parameters: |dE| [eV], |d log10 s|, |d log10 rho'|, |d fill|."""
+        """Errors against the true parameters.  TL: |dE| [eV], |d log10 s|,
+        |d log10 rho'|, |d fill|.  ISO: |d log10 k| (k = s e^{-E/kT}), |d fill|."""
         t, h = np.asarray(truth, float), self.theta
+        if self.kind == "iso":
+            return {"log10_k": abs(math.log10(iso_rate(h)[0] / iso_rate(t)[0])),
+                    "fill": abs(h[4] - t[4])}
         rp = lambda th: rho_prime_for(th[3], th[2])
         return {"E": abs(h[0] - t[0]), "log10_s": abs(math.log10(h[1] / t[1])),
                 "log10_rho_prime": abs(math.log10(rp(h) / rp(t))), "fill": abs(h[4] - t[4])}
 
 
 def judge_objective(ctx: Context, theta: np.ndarray, y: np.ndarray, scale: float,
-                    seed: int = DEFAULT_SEED + 999) -> float:
+                    seed: int = DEFAULT_SEED + 999, kind: S.CurveKind = S.TL) -> float:
     """Objective of one parameter set on a fresh Monte Carlo judge curve (64 clusters x
     1000 electrons, a seed no fit uses): the common yardstick for comparing fits."""
-    c = S.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)
+    c = kind.mc_curves(ctx, np.atleast_2d(theta), n=JUDGE_N, sims=JUDGE_SIMS, seed=seed)
     return float(objective(c, y, scale)[0])
 
 
-def _mc_eval(ctx: Context, th: np.ndarray, sims: int, n: int, seed: int) -> np.ndarray:
-    return S.mc_curves(ctx, th, n=n, sims=sims, seed=seed)
+def _mc_eval(ctx: Context, th: np.ndarray, sims: int, n: int, seed: int,
+             kind: S.CurveKind = S.TL) -> np.ndarray:
+    return kind.mc_curves(ctx, th, n=n, sims=sims, seed=seed)
 
 
-def fit_mc(ctx: Context, y: np.ndarray, scale: float, beta: float, pop: int = 64,
+def fit_mc(ctx: Context, y: np.ndarray, scale: float, fixed: float, pop: int = 64,
            gens: int = 30, sims: int = JUDGE_SIMS, n: int = JUDGE_N, seed: int = 0,
-           init: Optional[np.ndarray] = None, log: Optional[Callable[[str], None]] = None
-           ) -> FitResult:
-    """Plain Monte Carlo DE: every candidate simulated with sims clusters of n electrons."""
+           init: Optional[np.ndarray] = None, log: Optional[Callable[[str], None]] = None,
+           kind: S.CurveKind = S.TL) -> FitResult:
+    """Plain Monte Carlo DE: every candidate simulated with sims clusters of n electrons.
+    fixed: the heating rate (TL) or the temperature (ISO)."""
     t0 = time.time()
     rng = np.random.default_rng(seed)
     de = DifferentialEvolution(N_FIT, pop, rng, init=init)
     key = DEFAULT_SEED + 1000 * seed
-    de.seed_fitness(objective(_mc_eval(ctx, full_theta(de.x, beta), sims, n, key), y, scale))
+    de.seed_fitness(objective(_mc_eval(ctx, full_theta(de.x, fixed, kind), sims, n, key, kind),
+                              y, scale))
     hist = [de.best[1]]
     for g in range(gens):
         trial = de.propose()
-        f = objective(_mc_eval(ctx, full_theta(trial, beta), sims, n, key + g + 1), y, scale)
+        f = objective(_mc_eval(ctx, full_theta(trial, fixed, kind), sims, n, key + g + 1, kind),
+                      y, scale)
         de.accept(trial, f)
         hist.append(de.best[1])
         if log:
             log(f"mc gen {g + 1}: best {hist[-1]:.3e}")
     u, fbest = de.best
-    return FitResult(full_theta(u, beta)[0], fbest, float("nan"), time.time() - t0,
-                     pop * (gens + 1), 0, history=hist)
+    return FitResult(full_theta(u, fixed, kind)[0], fbest, float("nan"), time.time() - t0,
+                     pop * (gens + 1), 0, history=hist, kind=kind.name)
 
 
-def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, beta: float,
-                  pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
-                  fallback_pop: int = 64, fallback_gens: int = 20, patience: int = 25,
-                  log: Optional[Callable[[str], None]] = None) -> FitResult:
-    """Network-scored DE (population on the device), Monte Carlo judge of the best
-    candidate, Monte Carlo fallback when the network misses the judge by more than
-    gate (NRMSE).  Every `patience` generations the best objective is read back;
-    the search stops once it has improved by less than 1e-4 of itself since."""
-    t0 = time.time()
-    # the encoded fit coordinates are the network's inputs; beta is the sixth
-    u_beta = float(S.encode(full_theta(np.zeros(N_FIT), beta))[0, 5])
-    de = NetworkDE(net, y, scale, [u_beta], N_FIT, pop, seed)
+def _network_search(net: S.Network, y: np.ndarray, scale: float, u_fixed: float, pop: int,
+                    gens: int, seed: int, patience: int, peak: float):
+    """One network-scored DE; returns (search, best-objective history, generations)."""
+    de = NetworkDE(net, y, scale, [u_fixed], N_FIT, pop, seed, peak=peak)
+    hist = [de.best()[1]]
+    g = 0
     try:
-        hist = [de.best()[1]]
-        g = 0
         while g < gens:
             k = min(patience, gens - g)
             de.run(k)
@@ -255,24 +270,76 @@ def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, bet
             hist.append(de.best()[1])
             if hist[-1] > hist[-2] * (1.0 - 1e-4):
                 break
+    except Exception:
+        de.close()
+        raise
+    return de, hist, g
+
+
+def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, fixed: float,
+                  pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
+                  fallback_pop: int = 64, fallback_gens: int = 20, patience: int = 25,
+                  refine: int = 1, log: Optional[Callable[[str], None]] = None) -> FitResult:
+    """Network-scored DE (population on the device), Monte Carlo judge of the best
+    candidate, Monte Carlo fallback when the network misses the judge by more than
+    gate (NRMSE).  The curve kind is the network's (TL: fixed is the heating rate;
+    ISO: the temperature).  Every `patience` generations the best objective is read
+    back; the search stops once it has improved by less than 1e-4 of itself since.
+
+    refine rounds correct the network by what the judge saw: the search runs again
+    against y - scale (judge - network) at the last best candidate, the network's
+    error there taken as constant nearby (output space mapping), and its best is
+    judged with the same judge seed; the better judged candidate is kept.  Each
+    round costs one network search and one judge simulation."""
+    t0 = time.time()
+    kind = net.kind
+    peak = max(float(np.max(y)), 1.0)
+    # the encoded fit coordinates are the network's inputs; the fixed one is the sixth
+    u_fixed = float(kind.encode(full_theta(np.zeros(N_FIT), fixed, kind))[0, 5])
+    judge_seed = DEFAULT_SEED + 77 + seed
+    de, hist, g = _network_search(net, y, scale, u_fixed, pop, gens, seed, patience, peak)
+    n_net = pop * (g + 1)
+    try:
         u, _ = de.best()
-        th = full_theta(u, beta)
-        judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed)[0]
+        th = full_theta(u, fixed, kind)
+        judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, judge_seed, kind)[0]
         pred = S.predict_curves(net, th)[0]
         err = float(S.nrmse(pred, judge)[0])
         if log:
             log(f"surrogate fit: {g} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
-        n_net = pop * (g + 1)
-        if err <= gate:
-            return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
-                             1, n_net, False, hist)
-        xs, fs = de.population()        # the fallback's starting members
+        if err > gate:
+            xs, fs = de.population()        # the fallback's starting members
     finally:
         de.close()
+    if err <= gate:
+        best_th, best_judge, n_mc = th, judge, 1
+        best_f = float(objective(judge, y, scale)[0])
+        for r in range(refine):
+            y_corr = y - scale * (judge - pred)
+            de, h, g = _network_search(net, y_corr, scale, u_fixed, pop, gens, seed + 7919 * (r + 1),
+                                       patience, peak)
+            try:
+                u, _ = de.best()
+            finally:
+                de.close()
+            n_net += pop * (g + 1)
+            hist += h
+            th = full_theta(u, fixed, kind)
+            judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, judge_seed, kind)[0]
+            pred = S.predict_curves(net, th)[0]
+            n_mc += 1
+            f = float(objective(judge, y, scale)[0])
+            if log:
+                log(f"refine {r + 1}: {g} generations, judged objective {f:.3e} (was {best_f:.3e})")
+            if f < best_f:
+                best_th, best_judge, best_f = th, judge, f
+                err = float(S.nrmse(pred, judge)[0])
+        return FitResult(best_th[0], best_f, err, time.time() - t0, n_mc, n_net, False, hist,
+                         kind.name)
     # fallback: Monte Carlo generations from the network's best candidates
     import torch
     order = torch.argsort(fs)[:fallback_pop]
-    res = fit_mc(ctx, y, scale, beta, pop=fallback_pop, gens=fallback_gens, seed=seed + 1,
-                 init=xs[order].double().cpu().numpy(), log=log)
+    res = fit_mc(ctx, y, scale, fixed, pop=fallback_pop, gens=fallback_gens, seed=seed + 1,
+                 init=xs[order].double().cpu().numpy(), log=log, kind=kind)
     return FitResult(res.theta, res.objective_mc, err, time.time() - t0,
-                     1 + res.mc_candidates, n_net, True, hist + res.history)
+                     1 + res.mc_candidates, n_net, True, hist + res.history, kind.name)

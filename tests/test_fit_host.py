@@ -85,3 +85,52 @@ def test_run_table_is_byte_identical_to_the_per_set_runs():
     assert bytes(run_array(table)) == per_set
     arr = run_array(table)
     assert run_array(arr) is arr
+
+
+def test_iso_edges_are_log_time_bins():
+    e = S.iso_edges()
+    assert e[0] == 0 and e[-1] == S.ISO_STEPS and e.size == S.N_BINS + 1
+    d = np.diff(e)
+    assert np.all(d >= 1) and np.all(d[:S.ISO_LINEAR_BINS] == 1)
+    # geometric beyond the single steps: within half a step of L r^k, r = (4000 / L)^(1/209)
+    L = S.ISO_LINEAR_BINS
+    r = (S.ISO_STEPS / L) ** (1.0 / (S.N_BINS - L))
+    k = np.arange(S.N_BINS - L + 1)
+    assert np.all(np.abs(e[L:] - L * r ** k) <= 0.5 + 1e-9)
+    np.testing.assert_array_equal(np.asarray(S.ISO.edges), e)
+    t = S.ISO.axis()
+    assert t[0] == S.ISO_DT and t[-1] == 2000.0
+
+
+def test_iso_kind_box_and_run_table():
+    from mcluminescence_amd._abi import run_array
+    rng = np.random.default_rng(2)
+    th = S.ISO.sample_params(300, rng)
+    assert np.all(th[:, 5] >= 400) and np.all(th[:, 5] <= 600)
+    np.testing.assert_allclose(S.ISO.decode(S.ISO.encode(th)), th, rtol=1e-6)
+    per_set = bytes(run_array([S.ISO.run(t, 400, 8) for t in th]))
+    assert bytes(run_array(S.ISO.table(th, 400, 8))) == per_set
+    r = S.ISO.run(th[0], 400, 8)
+    assert r.T_rate == 0 and r.dt == S.ISO_DT and r.n_steps == S.ISO_STEPS
+    assert abs(r.T_start + 273.15 - th[0, 5]) < 1e-9
+
+
+def test_iso_recovery_is_in_the_rate_and_fill():
+    th = np.array([1.4, 1e13, 4e9, 1e23, 0.5, 500.0])
+    # another (E, s) with the same k at T_iso: no rate error
+    th2 = th.copy()
+    th2[0] = 1.5
+    th2[1] = th[1] * np.exp((th2[0] - th[0]) / (8.617343e-5 * 500.0))
+    r = F.FitResult(th2, 0.0, 0.0, 0.0, 0, 0, kind="iso").recovery(th)
+    assert set(r) == {"log10_k", "fill"} and r["log10_k"] < 1e-12
+    assert F.iso_rate(th)[0] == pytest.approx(0.07738, rel=1e-3)
+
+
+def test_iso_truths_resolve_their_decay():
+    from mcluminescence_amd import fit_bench
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        th = fit_bench.draw_truth_iso(rng)
+        tau = 1.0 / F.iso_rate(th)[0]
+        assert fit_bench.ISO_TIME_CONSTANT[0] <= tau <= fit_bench.ISO_TIME_CONSTANT[1]
+        assert 420 <= th[5] <= 580 and 0.3 <= th[4] <= 1.0
