@@ -295,6 +295,30 @@ int mcl_glow_curve(void* ctx, const mcl_run_params* runs, int32_t n_runs, const 
                    int64_t curve_stride, int32_t* curve_len, mcl_stats* stats);
 
 /*
+ * Allocation-free variant of mcl_run_fixed_dt, for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).  It has
+ * no counterpart in the reference.  Same runs, seed and n_bins as
+ * mcl_run_fixed_dt with MCL_F_DEVICE_PTRS (one workgroup per cluster, no
+ * persistent or graph launch) and the same counts and live electron-steps, but
+ * no allocation of any kind: the run descriptors, cluster map and work counters
+ * go to a static device array of MCL_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends), staged from a fixed host block.
+ *   counts_dev      DEVICE pointer [total_clusters][row_len] (caller-owned)
+ *   live_steps_dev  DEVICE pointer [total_clusters], or NULL
+ * A call that needs more scratch than that (192 B per run plus 40 B per
+ * cluster, each part rounded up to 256 B) fails with MCL_ERANGE before any
+ * device work.  Every call on one GPU uses that GPU's single scratch array, so
+ * calls that overlap on one GPU (several host threads or contexts) race; calls
+ * are synchronous on the context's stream, so calls from one thread never
+ * overlap.  Nothing in the scratch needs clearing: every call writes all of it
+ * that its kernel reads.
+ */
+#define MCL_NOALLOC_SCRATCH_BYTES (4u << 20)
+int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_runs,
+                             uint64_t seed, int32_t n_bins, uint32_t* counts_dev,
+                             uint64_t* live_steps_dev, mcl_stats* stats);
+
+/*
  * Counts of one cluster of a sweep: cluster (run_index, sim) of the sweep that
  * holds `run` at position run_index, identical to its row in mcl_run_fixed_dt,
  * without simulating the sweep's other clusters.  For the one-cluster object

@@ -157,6 +157,8 @@ _SIGNATURES = {
     "mcl_run_cluster": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_int32, C.c_uint64,
                                   C.c_int32, _P, _P]),
     "mcl_bin_counts": (C.c_int, [_P, _P, C.c_int64, C.c_int32, C.c_int64, _P, C.c_int32, _P]),
+    "mcl_run_fixed_dt_noalloc": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, C.c_uint64,
+                                           C.c_int32, _P, _P, C.POINTER(Stats)]),
     "mcl_run_event": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, C.c_uint64, C.c_int32,
                                 C.c_uint32, _P, _P, _P, _P, C.POINTER(Stats)]),
     "mcl_run_lab": (C.c_int, [_P, C.POINTER(RunParams), C.c_int32, _P, _P, C.c_uint32, C.c_uint64,
@@ -449,6 +451,20 @@ class Context:
         st = self.lib.mcl_bin_counts(self.handle, C.c_void_p(counts_ptr), int(n_sets), int(sims),
                                      int(row_stride), _ptr(e), len(e) - 1, C.c_void_p(out_ptr))
         self.check(st, "mcl_bin_counts")
+
+    def run_fixed_dt_noalloc(self, runs: Sequence[RunParams], counts_ptr: int,
+                             live_ptr: Optional[int] = None, seed: int = DEFAULT_SEED,
+                             n_bins: int = 0) -> "Stats":
+        """mcl_run_fixed_dt_noalloc: counts and live steps into caller-owned DEVICE
+        buffers (addresses), scratch from the library's static device array."""
+        arr = run_array(runs)
+        stats = Stats()
+        st = self.lib.mcl_run_fixed_dt_noalloc(self.handle, arr, len(runs), C.c_uint64(seed),
+                                               int(n_bins), C.c_void_p(counts_ptr),
+                                               C.c_void_p(live_ptr) if live_ptr else None,
+                                               C.byref(stats))
+        self.check(st, "mcl_run_fixed_dt_noalloc")
+        return stats
 
     def run_cluster(self, run: RunParams, run_index: int, sim: int, seed: int = DEFAULT_SEED,
                     n_bins: int = 0):

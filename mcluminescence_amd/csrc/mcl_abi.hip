@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -19,6 +20,45 @@
 #include "../../include/mcl.h"
 #include "mcl_ctx.h"
 #include "mcl_internal.h"
+
+// Device memory of mcl_run_fixed_dt_noalloc (run descriptors, cluster map and
+// per-cluster work counters of one call): a file-scope array, one copy per GPU,
+// held until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address comes from
+// hipGetSymbolAddress, once per device in mcl_create, and only that address is
+// used (kernel arguments, copies).
+static __device__ __align__(256) unsigned char scratch[MCL_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices] = {};   // device address per HIP device
+hipError_t g_scratch_err[kMaxScratchDevices] = {};
+
+// With `device` current: record the device address of its scratch copy.
+void init_scratch(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    if (e == hipSuccess) g_scratch_dev[device] = (unsigned char*)ptr;
+    else (void)hipGetLastError();        // reported by the entry point that needs it
+}
+
+unsigned char* scratch_for(int device, hipError_t* err)
+{
+    if (device < 0 || device >= kMaxScratchDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    *err = g_scratch_err[device];
+    return g_scratch_dev[device];
+}
+}  // namespace
 
 namespace mcl {
 
@@ -586,6 +626,13 @@ int mcl_create(void** ctx_out, int hip_device)
     c->stream = c->own_stream;
     c->n_cu = prop.multiProcessorCount;
     c->max_lds = (int)prop.sharedMemPerBlock;
+    if (hipEventCreate(&c->na_ev0) != hipSuccess || hipEventCreate(&c->na_ev1) != hipSuccess) {
+        if (c->na_ev0) hipEventDestroy(c->na_ev0);
+        hipStreamDestroy(c->own_stream);
+        delete c;
+        return MCL_EDEVICE;
+    }
+    init_scratch(hip_device);
     *ctx_out = c;
     return MCL_OK;
 }
@@ -598,6 +645,8 @@ void mcl_destroy(void* ctx)
         hipStreamSynchronize(c->own_stream);
         hipStreamDestroy(c->own_stream);
     }
+    if (c->na_ev0) hipEventDestroy(c->na_ev0);
+    if (c->na_ev1) hipEventDestroy(c->na_ev1);
     mcl::pool_clear(c);
     delete c;
 }
@@ -1182,6 +1231,155 @@ int mcl_run_fixed_dt(void* ctx, const mcl_run_params* runs, int32_t n_runs, uint
     mcl::pool_release(c, d_live, true);
     plan_free(p, true);
     return st;
+}
+
+int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_runs,
+                             uint64_t seed, int32_t n_bins, uint32_t* counts_dev,
+                             uint64_t* live_steps_dev, mcl_stats* stats)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return MCL_EINVAL;
+    if (!counts_dev) return fail(c, MCL_EINVAL, "counts is null");
+    if (!runs || n_runs <= 0) return fail(c, MCL_EINVAL, "no runs");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MCL_EDEVICE, "hipSetDevice failed");
+    hipError_t e = hipSuccess;
+    unsigned char* const base = scratch_for(c->device, &e);
+    if (!base)
+        return fail(c, MCL_EDEVICE, "no scratch address for device %d: %s", c->device,
+                    hipGetErrorString(e));
+    // sizes, as make_plan derives them
+    int64_t total = 0;
+    int max_e = 0, max_h = 0, max_cells = 1, max_steps = 0;
+    for (int r = 0; r < n_runs; ++r) {
+        mcl_geometry g;
+        const int st = validate_run(c, runs[r], r, n_bins, g);
+        if (st) return st;
+        total += runs[r].sims;
+        if (runs[r].sims > 0) {
+            max_e = std::max(max_e, g.n_e0);
+            max_h = std::max(max_h, g.n_h);
+            max_steps = std::max(max_steps, (int)runs[r].n_steps);
+            const int G = mcl::grid_dim(g.n_h, c->holes_per_cell);
+            max_cells = std::max(max_cells, G * G * G);
+        }
+    }
+    if (total <= 0) return fail(c, MCL_EINVAL, "no clusters (sum of sims is 0)");
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t n_cl = (size_t)total;
+    const size_t off_crun = up(sizeof(mcl::RunDesc) * (size_t)n_runs);
+    const size_t off_csim = off_crun + up(sizeof(int32_t) * n_cl);
+    const size_t off_work = off_csim + up(sizeof(int32_t) * n_cl);
+    const size_t need = off_work + sizeof(uint64_t) * 4 * n_cl;
+    if (need > (size_t)MCL_NOALLOC_SCRATCH_BYTES)
+        return fail(c, MCL_ERANGE,
+                    "%d runs / %lld clusters need %zu B of scratch; the allocation-free entry "
+                    "point has %u B (use mcl_run_fixed_dt)",
+                    n_runs, (long long)total, need, (unsigned)MCL_NOALLOC_SCRATCH_BYTES);
+    const int h_cap = ((max_h + 3) / 4) * 4 + 4;
+    const int e_cap = std::max(64, ((max_e + 63) / 64) * 64);
+    const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
+    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
+    if (lds > c->max_lds)
+        return fail(c, MCL_ERANGE,
+                    "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B",
+                    max_e, max_h, lds, c->max_lds);
+    // descriptors and cluster map through a fixed host staging block, one chunk
+    // at a time (each copied and waited for before the block is refilled)
+    alignas(64) unsigned char stage[16384];
+    auto upload = [&](size_t dst_off, size_t bytes) -> hipError_t {
+        hipError_t r = hipMemcpyAsync(base + dst_off, stage, bytes, hipMemcpyHostToDevice, c->stream);
+        if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+        return r;
+    };
+    {
+        const int per = (int)(sizeof stage / sizeof(mcl::RunDesc));
+        mcl::RunDesc* d = (mcl::RunDesc*)stage;
+        for (int r0 = 0; r0 < n_runs && e == hipSuccess; r0 += per) {
+            const int k = std::min(per, n_runs - r0);
+            for (int j = 0; j < k; ++j) {
+                mcl_geometry g;
+                geometry(runs[r0 + j], g);
+                d[j] = make_desc(runs[r0 + j], g, r0 + j, c->holes_per_cell);
+            }
+            e = upload(sizeof(mcl::RunDesc) * (size_t)r0, sizeof(mcl::RunDesc) * (size_t)k);
+        }
+    }
+    {
+        const size_t per = sizeof stage / (2 * sizeof(int32_t));
+        int32_t* cr = (int32_t*)stage;
+        int r = 0, s = 0;          // run and sim of the next cluster
+        for (size_t k0 = 0; k0 < n_cl && e == hipSuccess; k0 += per) {
+            const size_t k = std::min(per, n_cl - k0);
+            int32_t* cs = cr + k;
+            for (size_t j = 0; j < k; ++j) {
+                while (s >= runs[r].sims) {
+                    ++r;
+                    s = 0;
+                }
+                cr[j] = r;
+                cs[j] = s++;
+            }
+            e = upload(off_crun + sizeof(int32_t) * k0, sizeof(int32_t) * k);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(base + off_csim + sizeof(int32_t) * k0, cs, sizeof(int32_t) * k,
+                                   hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+    }
+    if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds, c->max_lds);
+    if (e != hipSuccess)
+        return fail(c, MCL_EDEVICE, "noalloc setup: %s", hipGetErrorString(e));
+    int waves = 1, per_cu = 0;
+    int st = choose_waves(c, lds, e_cap, total, 8, &waves, &per_cu);
+    if (st) return st;
+    mcl::LaunchArgs a{};
+    a.runs = (const mcl::RunDesc*)base;
+    a.c_run = (const int32_t*)(base + off_crun);
+    a.c_sim = (const int32_t*)(base + off_csim);
+    a.n_clusters = (int32_t)total;
+    a.order = nullptr;
+    a.n_order = (int32_t)total;
+    a.queue = nullptr;
+    a.key0 = (uint32_t)seed;
+    a.key1 = (uint32_t)(seed >> 32);
+    a.n_bins = n_bins;
+    a.row_len = n_bins > 0 ? n_bins : max_steps;
+    a.counts = counts_dev;
+    a.live_steps = live_steps_dev;
+    a.work = (uint64_t*)(base + off_work);
+    a.h_cap = h_cap;
+    a.e_cap = e_cap;
+    a.g_cap = g_cap;
+    a.wave_search_max = (int32_t)c->wave_search_max;
+    a.lookahead_max = (int32_t)c->lookahead_max;
+    a.dump.enabled = 0;
+    e = hipEventRecord(c->na_ev0, c->stream);
+    if (e == hipSuccess) e = mcl::launch_fixed_dt(a, (int)total, lds, waves, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->na_ev1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, MCL_EDEVICE, "noalloc launch: %s", hipGetErrorString(e));
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->na_ev0, c->na_ev1) == hipSuccess) stats->kernel_ms = ms;
+        stats->clusters = (uint64_t)total;
+        // the work counters back through the same staging block
+        const size_t per = sizeof stage / (4 * sizeof(uint64_t));
+        const uint64_t* w = (const uint64_t*)stage;
+        for (size_t k0 = 0; k0 < n_cl && e == hipSuccess; k0 += per) {
+            const size_t k = std::min(per, n_cl - k0);
+            e = hipMemcpyAsync(stage, base + off_work + 4 * sizeof(uint64_t) * k0,
+                               4 * sizeof(uint64_t) * k, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            for (size_t j = 0; j < k && e == hipSuccess; ++j) {
+                stats->live_electron_steps += w[4 * j];
+                stats->recombinations += w[4 * j + 2];
+                stats->cluster_steps += w[4 * j + 3];
+            }
+        }
+        if (e != hipSuccess) return fail(c, MCL_EDEVICE, "noalloc stats: %s", hipGetErrorString(e));
+    }
+    return MCL_OK;
 }
 
 int mcl_run_cluster(void* ctx, const mcl_run_params* run, int32_t run_index, int32_t sim,

@@ -28,6 +28,8 @@ struct Ctx {
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
     double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
     std::string err;
+    // timing events of mcl_run_fixed_dt_noalloc, made once by mcl_create
+    hipEvent_t na_ev0 = nullptr, na_ev1 = nullptr;
     // Device blocks of finished runs, kept for later runs (hipMalloc and hipFree,
     // which waits for the device, cost more than a small run's kernels).
     std::mutex pool_mu;
