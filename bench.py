@@ -371,6 +371,42 @@ def glow_extras(ctx, runs, counts, batch: int = 1024) -> dict:
     return out
 
 
+def iso_fit_extras(ctx) -> dict:
+    """The isothermal half of the fit workload: the ISO network's held-out NRMSE
+    (64 resolved decays against 1024-cluster Monte Carlo) and one surrogate-assisted
+    fit of a synthetic isothermal lab curve against the plain Monte Carlo fit."""
+    import numpy as np
+    from mcluminescence_amd import fit as F
+    from mcluminescence_amd import fit_bench
+    from mcluminescence_amd import surrogate as S
+    from mcluminescence_amd import train_surrogate as T
+    net = S.Network.load(ctx, fit_bench.MODELS["iso"], max_batch=4096)
+    theta = T.sample(S.ISO, 64, np.random.default_rng(2024), resolved_only=True)
+    mc = S.ISO.mc_curves(ctx, theta, n=400, sims=1024, seed=S.DEFAULT_SEED + 4243)
+    e = S.nrmse(S.predict_curves(net, theta), mc)
+    rng = np.random.default_rng(7)
+    truth = fit_bench.draw_truth_iso(rng)
+    y, scale = F.lab_curve(ctx, truth, rng=rng, kind=S.ISO)
+    F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=2, seed=9)   # loads kernels
+    F.fit_mc(ctx, y, scale, truth[5], pop=8, gens=1, seed=9, kind=S.ISO)
+    rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=150, seed=1)
+    rm = F.fit_mc(ctx, y, scale, truth[5], pop=128, gens=40, seed=1, kind=S.ISO)
+    jo = lambda th: F.judge_objective(ctx, th, y, scale, kind=S.ISO)
+    net.close()
+    return {"heldout_nrmse_median": float(np.median(e)),
+            "heldout_nrmse_p90": float(np.percentile(e, 90)),
+            "heldout_note": "64 resolved decays (1/k in [10^-1, 10^4.5] s) vs 1024-cluster MC",
+            "fit_seconds_surrogate": rs.seconds, "fit_seconds_mc": rm.seconds,
+            "fit_speedup": rm.seconds / rs.seconds, "fit_judge_nrmse": rs.judge_nrmse,
+            "fit_fallback": rs.fallback,
+            "fit_objective_surrogate": jo(rs.theta), "fit_objective_mc": jo(rm.theta),
+            "fit_objective_truth": jo(truth),
+            "recovery_surrogate": rs.recovery(truth), "recovery_mc": rm.recovery(truth),
+            "fit_note": "isothermal decay at T_iso = %.1f K, 1/k = %.1f s: surrogate DE 1024 x 150 "
+                        "generations + one 64 x 1000 Monte Carlo judge vs plain Monte Carlo DE "
+                        "128 x 40 generations" % (truth[5], 1.0 / F.iso_rate(truth)[0])}
+
+
 def surrogate_extras(ctx, fits: bool = True) -> dict:
     """Surrogate network: one AdamW training step at batch 4096 and one inference
     batch of 65,536 (device time, HIP events), and (fits) one surrogate-assisted fit
@@ -418,7 +454,8 @@ def surrogate_extras(ctx, fits: bool = True) -> dict:
     rs = F.fit_surrogate(ctx, net, y, scale, truth[5], pop=1024, gens=150, seed=1)
     rm = F.fit_mc(ctx, y, scale, truth[5], pop=128, gens=40, seed=1)
     net.close()
-    return {**timing,
+    iso = iso_fit_extras(ctx)
+    return {**timing, "iso": iso,
             "fit_seconds_surrogate": rs.seconds, "fit_seconds_mc": rm.seconds,
             "fit_speedup": rm.seconds / rs.seconds, "fit_judge_nrmse": rs.judge_nrmse,
             "fit_fallback": rs.fallback,

@@ -279,63 +279,33 @@ def _network_search(net: S.Network, y: np.ndarray, scale: float, u_fixed: float,
 def fit_surrogate(ctx: Context, net: S.Network, y: np.ndarray, scale: float, fixed: float,
                   pop: int = 1024, gens: int = 150, gate: float = 0.02, seed: int = 0,
                   fallback_pop: int = 64, fallback_gens: int = 20, patience: int = 25,
-                  refine: int = 1, log: Optional[Callable[[str], None]] = None) -> FitResult:
+                  log: Optional[Callable[[str], None]] = None) -> FitResult:
     """Network-scored DE (population on the device), Monte Carlo judge of the best
     candidate, Monte Carlo fallback when the network misses the judge by more than
     gate (NRMSE).  The curve kind is the network's (TL: fixed is the heating rate;
     ISO: the temperature).  Every `patience` generations the best objective is read
-    back; the search stops once it has improved by less than 1e-4 of itself since.
-
-    refine rounds correct the network by what the judge saw: the search runs again
-    against y - scale (judge - network) at the last best candidate, the network's
-    error there taken as constant nearby (output space mapping), and its best is
-    judged with the same judge seed; the better judged candidate is kept.  Each
-    round costs one network search and one judge simulation."""
+    back; the search stops once it has improved by less than 1e-4 of itself since."""
     t0 = time.time()
     kind = net.kind
     peak = max(float(np.max(y)), 1.0)
     # the encoded fit coordinates are the network's inputs; the fixed one is the sixth
     u_fixed = float(kind.encode(full_theta(np.zeros(N_FIT), fixed, kind))[0, 5])
-    judge_seed = DEFAULT_SEED + 77 + seed
     de, hist, g = _network_search(net, y, scale, u_fixed, pop, gens, seed, patience, peak)
     n_net = pop * (g + 1)
     try:
         u, _ = de.best()
         th = full_theta(u, fixed, kind)
-        judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, judge_seed, kind)[0]
+        judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, DEFAULT_SEED + 77 + seed, kind)[0]
         pred = S.predict_curves(net, th)[0]
         err = float(S.nrmse(pred, judge)[0])
         if log:
             log(f"surrogate fit: {g} generations, best {hist[-1]:.3e}, judge NRMSE {err:.4f}")
-        if err > gate:
-            xs, fs = de.population()        # the fallback's starting members
+        if err <= gate:
+            return FitResult(th[0], float(objective(judge, y, scale)[0]), err, time.time() - t0,
+                             1, n_net, False, hist, kind.name)
+        xs, fs = de.population()        # the fallback's starting members
     finally:
         de.close()
-    if err <= gate:
-        best_th, best_judge, n_mc = th, judge, 1
-        best_f = float(objective(judge, y, scale)[0])
-        for r in range(refine):
-            y_corr = y - scale * (judge - pred)
-            de, h, g = _network_search(net, y_corr, scale, u_fixed, pop, gens, seed + 7919 * (r + 1),
-                                       patience, peak)
-            try:
-                u, _ = de.best()
-            finally:
-                de.close()
-            n_net += pop * (g + 1)
-            hist += h
-            th = full_theta(u, fixed, kind)
-            judge = _mc_eval(ctx, th, JUDGE_SIMS, JUDGE_N, judge_seed, kind)[0]
-            pred = S.predict_curves(net, th)[0]
-            n_mc += 1
-            f = float(objective(judge, y, scale)[0])
-            if log:
-                log(f"refine {r + 1}: {g} generations, judged objective {f:.3e} (was {best_f:.3e})")
-            if f < best_f:
-                best_th, best_judge, best_f = th, judge, f
-                err = float(S.nrmse(pred, judge)[0])
-        return FitResult(best_th[0], best_f, err, time.time() - t0, n_mc, n_net, False, hist,
-                         kind.name)
     # fallback: Monte Carlo generations from the network's best candidates
     import torch
     order = torch.argsort(fs)[:fallback_pop]

@@ -2,8 +2,8 @@
 
     python -m mcluminescence_amd.train_surrogate --curves 200000 --epochs 200 \\
         --out mcluminescence_amd/surrogate_tl.npz
-    python -m mcluminescence_amd.train_surrogate --kind iso --curves 200000 \\
-        --epochs 300 --out mcluminescence_amd/surrogate_iso.npz
+    python -m mcluminescence_amd.train_surrogate --kind iso --curves 200000 --sims 32 \\
+        --epochs 400 --out mcluminescence_amd/surrogate_iso.npz
 
 1. samples parameter sets in the encoded box of the kind (surrogate.RANGES for
    TL glow curves, surrogate.ISO_RANGES for isothermal decays): uniformly, and for
