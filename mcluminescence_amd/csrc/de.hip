@@ -52,7 +52,6 @@ __device__ __forceinline__ uint32_t below(uint32_t w, uint32_t n)
 }
 
 // the hidden layers' activation, as the network's GEMM epilogue computes it
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 __global__ __launch_bounds__(256) void de_init_kernel(DeArgs a)
 {

@@ -712,6 +712,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->waves_per_cluster = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "mlp_fused") == 0) {
+        if (!(value == 0.0 || value == 1.0))
+            return fail(c, MCL_EINVAL, "mlp_fused must be 0 or 1 (got %g)", value);
+        c->mlp_fused = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -737,6 +743,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "waves_per_cluster") == 0) {
         *value = c->waves_per_cluster;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "mlp_fused") == 0) {
+        *value = c->mlp_fused;
         return MCL_OK;
     }
     return MCL_EINVAL;

@@ -27,6 +27,7 @@ struct Ctx {
     double wave_search_max = 32.0; // queue length searched by the whole wave
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
     double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
+    double mlp_fused = 1.0;        // surrogate inference as one kernel (1) or per layer (0)
     std::string err;
     // timing events of mcl_run_fixed_dt_noalloc, made once by mcl_create
     hipEvent_t na_ev0 = nullptr, na_ev1 = nullptr;

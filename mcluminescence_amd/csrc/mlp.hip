@@ -28,7 +28,6 @@ constexpr int kThreads = 256;
 constexpr int kBK = 64;    // K per LDS stage
 constexpr int kPad = 8;    // LDS row padding [bf16]: 80-byte rows, conflict-free 16-byte reads
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 __device__ __forceinline__ float wave_sum(float v)
 {

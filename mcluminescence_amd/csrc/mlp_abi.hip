@@ -45,6 +45,13 @@ struct Mlp {
     float* loss_tmp = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    // the fused inference kernel's network: weights in fragment order (repacked
+    // after any change of the weights), biases, LDS bytes; usable iff fused_ok
+    M::DeNet fnet{};
+    bf16* wf = nullptr;
+    bool wf_stale = true;
+    bool fused_ok = false;
+    int fused_lds = 0;
 
     ~Mlp()
     {
@@ -175,10 +182,26 @@ int forward_packed(Mlp* m, int32_t B, float* y)
     return MCL_OK;
 }
 
-// The forward pass of batch B: pack x into m->X0, then the layers.
+// The forward pass of batch B: one kernel for the whole network where its shape
+// allows (the context's "mlp_fused" knob, default on), else pack x into m->X0 and
+// run the layers; the two give identical outputs.
 int forward(Mlp* m, const float* x, int32_t B, float* y)
 {
-    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, m->ctx->stream));
+    hipStream_t st = m->ctx->stream;
+    if (m->fused_ok && m->ctx->mlp_fused != 0.0) {
+        if (m->wf_stale) {
+            int64_t off = 0;
+            for (size_t l = 0; l < m->L.size(); ++l) {
+                const Layer& s = m->L[l];
+                HIP_TRY(m->ctx, M::de_pack(s.wb, s.out, s.kp, m->wf + off, st));
+                off += (int64_t)s.out * s.kp;
+            }
+            m->wf_stale = false;
+        }
+        HIP_TRY(m->ctx, M::infer_fused(x, B, m->d.n_in, m->fnet, y, m->d.n_out, m->fused_lds, st));
+        return MCL_OK;
+    }
+    HIP_TRY(m->ctx, M::pack_input(x, nullptr, m->d.n_in, B, m->L[0].kp, m->X0, nullptr, st));
     return forward_packed(m, B, y);
 }
 
@@ -438,6 +461,29 @@ int mcl_mlp_create(void* ctxp, const mcl_mlp_desc* desc, void** out)
         TRY(dalloc(m, &m->loss_tmp, 1));
         HIP_TRY(c, hipEventCreate(&m->ev0));
         HIP_TRY(c, hipEventCreate(&m->ev1));
+        // the fused inference kernel's view of the network
+        int64_t nwf = 0, widest = 0;
+        for (size_t l = 0; l < m->L.size(); ++l) {
+            const Layer& s = m->L[l];
+            m->fnet.L[l] = M::DeLayer{nullptr, s.bw, s.out, s.kp};
+            nwf += (int64_t)s.out * s.kp;
+            widest = std::max<int64_t>(widest, s.kp);
+            if (l + 1 < m->L.size()) widest = std::max<int64_t>(widest, s.out);
+        }
+        m->fnet.n_layers = (int)m->L.size();
+        m->fnet.lda = (int)widest + 8;
+        m->fused_lds = M::infer_fused_lds(m->fnet);
+        if (M::infer_fused_ok(m->fnet) && m->fused_lds <= c->max_lds &&
+            M::infer_fused_prepare(m->fused_lds) == hipSuccess) {
+            TRY(dalloc(m, &m->wf, nwf));
+            int64_t off = 0;
+            for (size_t l = 0; l < m->L.size(); ++l) {
+                m->fnet.L[l].w = m->wf + off;
+                off += (int64_t)m->L[l].out * m->L[l].kp;
+            }
+            m->fused_ok = true;
+        }
+        (void)hipGetLastError();
         return MCL_OK;
     };
     st = run();
@@ -504,6 +550,7 @@ int mcl_mlp_set_params(void* mlp, const float* params)
         HIP_TRY(c, hipMemset(s.bv, 0, s.out * sizeof(float)));
     }
     TRY(run_optimizer(m, M::kOptRefresh, 64, {}, nullptr, nullptr));
+    m->wf_stale = true;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MCL_OK;
 }
@@ -544,6 +591,7 @@ int mcl_mlp_train_step(void* mlp, const float* x, const float* t, const int32_t*
     std::vector<int> splits;
     TRY(forward_backward(m, x, t, idx, B, loss, splits));
     TRY(run_optimizer(m, M::kOptAdamW, B, splits, nullptr, hp));
+    m->wf_stale = true;
     return end(m);
 }
 

@@ -8,6 +8,12 @@
 namespace mcl {
 namespace mlp {
 
+// The logistic function of the hidden layers' SiLU, z * sigmoid(z), as every
+// kernel computes it (GEMM epilogues, the fused forward, the DE generation):
+// the hardware reciprocal (1 ulp) of 1 + e^{-z}; one shared definition keeps
+// their results bit-identical.
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
 // What a GEMM's epilogue does with the FP32 accumulator acc[m][n] of
 // C = A B^T (A [M][K], B [N][K], both K-contiguous BF16).
 enum Epi : int {
@@ -136,6 +142,16 @@ int de_fused_lds(const DeNet& net, int n_out);
 hipError_t de_fused_prepare(int lds_bytes);
 hipError_t de_generation_fused(const DeArgs& a, const DeNet& net, uint32_t gen, int lds_bytes,
                                hipStream_t stream);
+
+// The inference forward in one kernel (mlp_fused.hip): 128 rows per workgroup
+// through every layer, activations in LDS, weights in fragment order (de_pack).
+// Hidden layers 512 wide (the first with its inputs padded to 64), the output
+// layer 256 or 512 wide.
+bool infer_fused_ok(const DeNet& net);
+int infer_fused_lds(const DeNet& net);
+hipError_t infer_fused_prepare(int lds_bytes);
+hipError_t infer_fused(const float* x, int B, int n_in, const DeNet& net, float* y, int n_out,
+                       int lds_bytes, hipStream_t stream);
 
 }  // namespace mlp
 }  // namespace mcl

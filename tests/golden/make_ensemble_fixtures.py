@@ -7,7 +7,7 @@ then into coarser bins; the fixture keeps the parameter set, the bin width and,
 per bin, the mean and standard deviation over the clusters of the
 recombinations per cluster.  Writes tests/golden/ensemble_event_mode.json.
 
-Run: python tests/golden/make_ensemble_fixtures.py   (about a minute on 8 cores)
+Run: python tests/golden/make_ensemble_fixtures.py   (a few minutes on 8 cores)
 """
 import json
 import sys
@@ -33,7 +33,7 @@ def cases():
     tun = W.tunnelling_tl12(sims=1, n=400, T_rate=1.0)[0]   # TL12 + basicTL12, 400 electrons
     return {
         # name: (run, sims, steps per coarse bin)
-        "iso_500K_400": (iso, 256, 10),
+        "iso_500K_400": (iso, 2048, 10),
         "tl_glow_500": (tl, 128, 50),
         "tl12_tunnelling_400": (tun, 128, 20),
     }

@@ -188,3 +188,28 @@ def test_close_right_after_unsynchronised_training(gpu_ctx):
     np.testing.assert_array_equal(fresh.forward(x).cpu().numpy(), ref.forward(x).cpu().numpy())
     fresh.close()
     ref.close()
+
+
+def test_fused_inference_matches_the_layered_form(gpu_ctx):
+    # one kernel for the whole forward (activations in LDS) against the per-layer
+    # GEMM kernels: the same products in the same order, bit for bit; a batch that
+    # ends in half a workgroup of rows, and after a training step (weights repacked)
+    import torch
+    print("fused vs layered inference, batch 65536 and 4160", flush=True)
+    net = _net(gpu_ctx, max_batch=65536, seed=4)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = (torch.rand((65536, 6), generator=g) * 2 - 1).cuda()
+    t = torch.rand((4096, 256), generator=g).cuda()
+    try:
+        for step in range(2):
+            outs = {}
+            for fused in (1.0, 0.0):
+                gpu_ctx.tune_set("mlp_fused", fused)
+                outs[fused] = (net.forward(x).cpu(), net.forward(x[:4160]).cpu())
+            for a, b in zip(outs[1.0], outs[0.0]):
+                torch.testing.assert_close(a, b, rtol=0, atol=0)
+            idx = torch.arange(4096, dtype=torch.int32, device="cuda")
+            net.train_step(x, t, idx, AdamW(1e-3, 0.9, 0.999, 1e-8, 0.0, step + 1))
+    finally:
+        gpu_ctx.tune_set("mlp_fused", 1.0)
+    net.close()
