@@ -106,7 +106,13 @@ const char* mcl_last_error(const void* ctx);
 /* Launch subsequent work on an existing HIP stream (e.g. torch's current
  * stream); NULL restores the context's own stream. */
 int         mcl_set_stream(void* ctx, void* hip_stream);
-/* Largest cluster (electrons, holes) one workgroup can hold in LDS. */
+/* Largest cluster (electrons, holes) one workgroup can hold in LDS.  Fixed-dt
+ * runs (mcl_run_fixed_dt, plans, mcl_run_cluster, mcl_cluster_state) take larger
+ * clusters, up to 32768 electrons and 65534 holes, in a global-memory form: holes
+ * and cell starts in a scratch slice per resident workgroup (16 B per hole slot,
+ * 4 B per cell), electrons in LDS while 16 B each fit; same results, lower
+ * speed.  Event-mode and lab runs, and mcl_run_fixed_dt_noalloc, stay within
+ * this limit. */
 int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_holes);
 /* Kernel tuning knobs of a context, applied to plans made afterwards.  Keys:
  *   "holes_per_cell"  mean holes per cell of each cluster's nearest-hole grid
@@ -121,9 +127,14 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     wave tests the next 64 steps at once and skips those in
  *                     which no electron can fire (default 8; 0 turns it off).
  *   "waves_per_cluster" wavefronts per fixed-dt cluster: 1, 2, 4 or 8 (the waves
- *                     split the cluster's 64-electron rows), or 0 (default):
- *                     each launch, or each size class of a persistent launch,
- *                     takes the count that keeps the most waves resident.
+ *                     split the cluster's 64-electron rows, at most 64 rows a
+ *                     wave, so clusters above 4096 electrons take at least the
+ *                     waves that cover them), or 0 (default): each launch, or
+ *                     each size class of a persistent launch, takes the count
+ *                     that keeps the most waves resident.
+ *   "global_memory"   1: fixed-dt plans keep every cluster's holes in the
+ *                     global-memory form (results identical, slower); 0
+ *                     (default): only clusters that do not fit LDS.
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

@@ -143,6 +143,15 @@ struct Plan {
     int32_t max_e = 0, max_h = 0;
     int32_t waves = 1;            // wavefronts per cluster of the one-workgroup-per-cluster launch
     uint32_t flags = 0;
+    // clusters beyond one workgroup's LDS (global-memory form): the plain launch is
+    // persistent, ggrid workgroups with a scratch slice of gstride bytes each
+    bool global = false;
+    int32_t e_lds = 0;
+    int64_t gstride = 0;
+    int32_t ggrid = 0;
+    size_t class_gbytes = 0;               // slices of the global-form classes
+    unsigned char* d_gscratch = nullptr;   // slices of the plain launch or of the classes
+    uint32_t* d_gticket = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     // persistent launch: one class kernel per cluster-size class
@@ -151,6 +160,10 @@ struct Plan {
         int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0, grid = 0;
         int32_t max_e = 0, max_h = 0;
         int32_t waves = 1;
+        bool global = false;            // global-memory form: scratch at goff, gstride a slice
+        int32_t e_lds = 0;
+        int64_t gstride = 0;
+        size_t goff = 0;
     };
     std::vector<Class> classes;
     int32_t* d_order = nullptr;
@@ -277,6 +290,8 @@ void plan_free(Plan* p, bool idle)
     mcl::pool_release(c, p->d_work, true);
     mcl::pool_release(c, p->d_order, true);
     mcl::pool_release(c, p->d_tickets, true);
+    mcl::pool_release(c, p->d_gscratch, true);
+    mcl::pool_release(c, p->d_gticket, true);
     if (p->gexec) hipGraphExecDestroy(p->gexec);
     for (hipStream_t s : p->side) hipStreamDestroy(s);
     for (hipEvent_t e : p->join) hipEventDestroy(e);
@@ -294,21 +309,31 @@ void plan_free(Plan* p, bool idle)
 // measured 2-20 % faster than four on launches of 16-64 clusters; with 5, four are
 // 20 % faster than two).  Size classes that run side by side
 // take at most four, so that a class of large clusters leaves wave slots on its
-// CUs to the others (eight measured 20 % slower on an uneven sweep).  per_cu:
-// that kernel's workgroups per CU.
+// CUs to the others (eight measured 20 % slower on an uneven sweep).  A wave
+// keeps per-row flags (candidates, stale electrons, losers) in 64-bit masks, so it
+// takes at most 64 rows: clusters above 4096 electrons get at least the waves
+// that cover their rows (the knob included), above 32768 none do (MCL_ERANGE).
+// per_cu: that kernel's workgroups per CU (global: of the global-memory kernels).
 int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int max_waves, int* waves,
-                 int* per_cu)
+                 int* per_cu, bool global)
 {
-    const int forced = (int)c->waves_per_cluster;
     const int rows = e_cap / 64;                            // 64-slot rows
-    int best = 1, best_pc = 0;
+    int need = 1;
+    while (need < 8 && (rows + need - 1) / need > 64) need *= 2;
+    if ((rows + need - 1) / need > 64)
+        return fail(c, MCL_ERANGE, "clusters of %d electrons: a cluster holds at most %d (64 rows "
+                    "of 64 slots for each of 8 waves)", e_cap, 8 * 64 * 64);
+    const int forced = c->waves_per_cluster != 0 ? std::max((int)c->waves_per_cluster, need) : 0;
+    int best = need, best_pc = 0;
     int64_t best_res = -1;
     for (int w : {1, 2, 4, 8}) {
+        if (w < need) continue;
         if (forced != 0 && w != forced) continue;
-        if (forced == 0 && (w > max_waves || w > rows)) break;
+        if (forced == 0 && w > need && (w > max_waves || w > rows)) break;
         if (lds + mcl::fixed_dt_waves_lds(w) > c->max_lds) continue;
         int pc = 0;
-        const hipError_t e = mcl::fixed_dt_occupancy(lds, w, &pc);
+        const hipError_t e = global ? mcl::fixed_dt_global_occupancy(lds, w, &pc)
+                                    : mcl::fixed_dt_occupancy(lds, w, &pc);
         if (e != hipSuccess) return fail(c, MCL_EDEVICE, "occupancy query: %s", hipGetErrorString(e));
         const int64_t res = (int64_t)w * std::min<int64_t>(count, (int64_t)std::max(pc, 1) * c->n_cu);
         if (res > best_res) {
@@ -317,9 +342,37 @@ int choose_waves(Ctx* c, int lds, int e_cap, int64_t count, int max_waves, int* 
             best_res = res;
         }
     }
+    if (best_res < 0)
+        return fail(c, MCL_ERANGE, "no workgroup of %d or more waves fits %d B of LDS per cluster",
+                    need, lds);
     *waves = best;
     *per_cu = best_pc;
     return MCL_OK;
+}
+
+// Where the clusters of given capacities live: wholly in LDS when
+// fixed_dt_lds_bytes and the largest exchange block fit a workgroup (and the
+// global_memory knob is off); otherwise
+// the global-memory form, with the electron arrays in LDS when they fit beside
+// the largest exchange block.
+// lds: the launch's dynamic LDS before the per-wave extra.
+struct Layout {
+    bool global = false;
+    int32_t e_lds = 0;
+    int32_t lds = 0;
+    int64_t gstride = 0;
+};
+
+Layout layout_for(const Ctx* c, int h_cap, int e_cap, int g_cap)
+{
+    Layout L;
+    L.lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
+    if (L.lds + mcl::fixed_dt_waves_lds(8) <= c->max_lds && c->global_memory == 0.0) return L;
+    L.global = true;
+    L.e_lds = mcl::fixed_dt_global_lds(e_cap, true) + mcl::fixed_dt_waves_lds(8) <= c->max_lds;
+    L.lds = mcl::fixed_dt_global_lds(e_cap, L.e_lds != 0);
+    L.gstride = mcl::fixed_dt_global_stride(h_cap, e_cap, g_cap, L.e_lds != 0);
+    return L;
 }
 
 // Persistent plans: bucket the clusters by electron count (powers of two from
@@ -369,23 +422,32 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
         cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
         cl.e_cap = std::max(64, ((cl.max_e + 63) / 64) * 64);
         cl.g_cap = ((max_cells + 1 + 3) / 4) * 4;
-        cl.lds = mcl::fixed_dt_lds_bytes(cl.h_cap, cl.e_cap, cl.g_cap);
-        if (cl.lds > c->max_lds)
-            return fail(c, MCL_ERANGE, "cluster of %d electrons / %d holes needs %d B of LDS; a "
-                        "workgroup has %d B", cl.max_e, cl.max_h, cl.lds, c->max_lds);
+        const Layout L = layout_for(c, cl.h_cap, cl.e_cap, cl.g_cap);
+        cl.lds = L.lds;
+        cl.global = L.global;
+        cl.e_lds = L.e_lds;
+        cl.gstride = L.gstride;
         // (the kernels' dynamic-LDS limit was raised to the plan's largest class)
         int per_cu = 0;
         const int st = choose_waves(c, cl.lds, cl.e_cap, cl.count,
-                                    p->classes.size() > 1 ? 4 : 8, &cl.waves, &per_cu);
+                                    p->classes.size() > 1 ? 4 : 8, &cl.waves, &per_cu, cl.global);
         if (st) return st;
         const double frac = c->persist_frac;
         int grid = (int)std::max(1.0, std::floor((double)std::max(per_cu, 1) * c->n_cu * frac));
         cl.grid = std::min(grid, cl.count);
     }
+    // one scratch block for the slices of every global-memory class
+    size_t gbytes = 0;
+    for (Plan::Class& cl : p->classes) {
+        if (!cl.global) continue;
+        cl.goff = gbytes;
+        gbytes += (size_t)cl.grid * (size_t)cl.gstride;
+    }
     const size_t ncls = p->classes.size();
     hipError_t e = mcl::pool_alloc(c, (void**)&p->d_order, sizeof(int32_t) * order.size());
     if (e == hipSuccess)
         e = mcl::pool_alloc(c, (void**)&p->d_tickets, sizeof(uint32_t) * std::max<size_t>(ncls, 1));
+    p->class_gbytes = gbytes;      // allocated with the plain launch's slices (make_plan)
     if (e == hipSuccess)
         e = hipMemcpy(p->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
@@ -432,11 +494,8 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     const int h_cap = ((max_h + 3) / 4) * 4 + 4;
     const int e_cap = std::max(64, ((max_e + 63) / 64) * 64);
     const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
-    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
-    if (lds > c->max_lds)
-        return fail(c, MCL_ERANGE,
-                    "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B",
-                    max_e, max_h, lds, c->max_lds);
+    const Layout L = layout_for(c, h_cap, e_cap, g_cap);
+    const int lds = L.lds;
     std::vector<int32_t> crun((size_t)total), csim((size_t)total);
     int64_t k = 0;
     for (int r = 0; r < n_runs; ++r)
@@ -459,6 +518,9 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->max_e = max_e;
     p->max_h = max_h;
     p->flags = flags;
+    p->global = L.global;
+    p->e_lds = L.e_lds;
+    p->gstride = L.gstride;
     hipError_t e = mcl::pool_alloc(c, (void**)&p->d_runs, sizeof(mcl::RunDesc) * (size_t)n_runs);
     if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_crun, sizeof(int32_t) * (size_t)total);
     if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_csim, sizeof(int32_t) * (size_t)total);
@@ -476,15 +538,18 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipEventCreate(&p->ev0);
     if (e == hipSuccess) e = hipEventCreate(&p->ev1);
-    if (e == hipSuccess) e = mcl::prepare_fixed_dt(lds, c->max_lds);
+    // (LDS limits: the plan's largest need, at most a workgroup's; classes need less)
+    if (e == hipSuccess)
+        e = mcl::prepare_fixed_dt(mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap), c->max_lds);
+    if (e == hipSuccess && L.global) e = mcl::prepare_fixed_dt_global(c->max_lds);
     if (e != hipSuccess) {
         plan_free(p, false);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "plan setup: %s",
                     hipGetErrorString(e));
     }
+    int per_cu = 0;
     {
-        int per_cu = 0;
-        const int st = choose_waves(c, lds, e_cap, total, 8, &p->waves, &per_cu);
+        const int st = choose_waves(c, lds, e_cap, total, 8, &p->waves, &per_cu, L.global);
         if (st) {
             plan_free(p, true);
             return st;
@@ -495,6 +560,21 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
         if (st) {
             plan_free(p, true);
             return st;
+        }
+    }
+    // global-memory form: one scratch slice per resident workgroup of the plain
+    // launch (its grid), or of the classes, and the plain launch's ticket
+    if (L.global) {
+        const int64_t resident = (int64_t)std::max(per_cu, 1) * c->n_cu;
+        p->ggrid = (int32_t)std::min<int64_t>(total, std::max<int64_t>(resident, 1));
+        const size_t plain = (size_t)p->ggrid * (size_t)p->gstride;
+        const size_t need = std::max(plain, p->class_gbytes);
+        if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_gscratch, need);
+        if (e == hipSuccess) e = mcl::pool_alloc(c, (void**)&p->d_gticket, sizeof(uint32_t));
+        if (e != hipSuccess) {
+            plan_free(p, true);
+            return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE,
+                        "global-memory scratch (%zu B): %s", need, hipGetErrorString(e));
         }
     }
     *out = p;
@@ -545,7 +625,14 @@ hipError_t enqueue_persistent(Plan* p, uint32_t* counts, uint64_t* live, hipStre
         a.h_cap = cl.h_cap;
         a.e_cap = cl.e_cap;
         a.g_cap = cl.g_cap;
-        if (e == hipSuccess) e = mcl::launch_fixed_dt(a, cl.grid, cl.lds, cl.waves, sk);
+        if (cl.global) {
+            a.gscratch = p->d_gscratch + cl.goff;
+            a.gstride = cl.gstride;
+            a.e_lds = cl.e_lds;
+            if (e == hipSuccess) e = mcl::launch_fixed_dt_global(a, cl.grid, cl.lds, cl.waves, sk);
+        } else if (e == hipSuccess) {
+            e = mcl::launch_fixed_dt(a, cl.grid, cl.lds, cl.waves, sk);
+        }
         if (e == hipSuccess && k > 0) e = hipEventRecord(p->join[(size_t)k], sk);
         if (e == hipSuccess && k > 0) e = hipStreamWaitEvent(s, p->join[(size_t)k], 0);
     }
@@ -594,7 +681,18 @@ int plan_launch(Plan* p, uint32_t* counts, uint64_t* live, const mcl::StateDump*
     if (dump) a.dump = *dump;
     HIP_TRY(c, hipEventRecord(p->ev0, c->stream));
     // (state dumps run one wave per cluster)
-    HIP_TRY(c, mcl::launch_fixed_dt(a, (int)p->total, p->lds, dump ? 1 : p->waves, c->stream));
+    if (p->global) {
+        // persistent over the plan's clusters in order, one scratch slice per workgroup
+        a.queue = p->d_gticket;
+        a.gscratch = p->d_gscratch;
+        a.gstride = p->gstride;
+        a.e_lds = p->e_lds;
+        const int grid = (int)std::min<int64_t>(p->ggrid, p->total);
+        HIP_TRY(c, mcl::launch_reset_tickets(p->d_gticket, 1, c->stream));
+        HIP_TRY(c, mcl::launch_fixed_dt_global(a, grid, p->lds, dump ? 1 : p->waves, c->stream));
+    } else {
+        HIP_TRY(c, mcl::launch_fixed_dt(a, (int)p->total, p->lds, dump ? 1 : p->waves, c->stream));
+    }
     HIP_TRY(c, hipEventRecord(p->ev1, c->stream));
     p->timed = true;
     return MCL_OK;
@@ -718,6 +816,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->mlp_fused = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "global_memory") == 0) {
+        if (!(value == 0.0 || value == 1.0))
+            return fail(c, MCL_EINVAL, "global_memory must be 0 or 1 (got %g)", value);
+        c->global_memory = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -747,6 +851,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "mlp_fused") == 0) {
         *value = c->mlp_fused;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "global_memory") == 0) {
+        *value = c->global_memory;
         return MCL_OK;
     }
     return MCL_EINVAL;
@@ -780,11 +888,11 @@ int mcl_plan_info_get(const void* plan, mcl_plan_info* info)
     info->total_clusters = p->total;
     info->n_bins = p->n_bins;
     info->row_len = p->row_len;
-    info->lds_bytes = p->lds;
+    info->lds_bytes = p->lds + mcl::fixed_dt_waves_lds(p->waves);
     info->max_electrons = p->max_e;
     info->max_holes = p->max_h;
-    info->grid = (int32_t)p->total;
-    info->block = mcl::kWaveHost;
+    info->grid = p->global ? (int32_t)std::min<int64_t>(p->ggrid, p->total) : (int32_t)p->total;
+    info->block = mcl::kWaveHost * p->waves;
     return MCL_OK;
 }
 
@@ -1291,7 +1399,8 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
     if (lds > c->max_lds)
         return fail(c, MCL_ERANGE,
-                    "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B",
+                    "cluster of %d electrons / %d holes needs %d B of LDS; a workgroup has %d B "
+                    "(larger clusters run in global memory: use mcl_run_fixed_dt)",
                     max_e, max_h, lds, c->max_lds);
     // descriptors and cluster map through a fixed host staging block, one chunk
     // at a time (each copied and waited for before the block is refilled)
@@ -1340,7 +1449,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     if (e != hipSuccess)
         return fail(c, MCL_EDEVICE, "noalloc setup: %s", hipGetErrorString(e));
     int waves = 1, per_cu = 0;
-    int st = choose_waves(c, lds, e_cap, total, 8, &waves, &per_cu);
+    int st = choose_waves(c, lds, e_cap, total, 8, &waves, &per_cu, false);
     if (st) return st;
     mcl::LaunchArgs a{};
     a.runs = (const mcl::RunDesc*)base;

@@ -28,6 +28,8 @@ struct Ctx {
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
     double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
     double mlp_fused = 1.0;        // surrogate inference as one kernel (1) or per layer (0)
+    double global_memory = 0.0;    // fixed-dt clusters in the global-memory form: 0 when
+                                   // they exceed LDS, 1 always
     std::string err;
     // timing events of mcl_run_fixed_dt_noalloc, made once by mcl_create
     hipEvent_t na_ev0 = nullptr, na_ev1 = nullptr;

@@ -66,6 +66,12 @@ struct LaunchArgs {
     int32_t wave_search_max;  // queues up to this long are searched one electron per wave (<= 64)
     int32_t lookahead_max;    // with at most this many live electrons, quiet steps are skipped
     StateDump dump;
+    // global-memory form (clusters beyond one workgroup's LDS): workgroup b keeps its
+    // cluster's holes and cell starts (and, unless e_lds, its electron arrays) in
+    // gscratch + b * gstride
+    unsigned char* gscratch;
+    int64_t gstride;
+    int32_t e_lds;            // electron arrays in LDS after the exchange slots
 };
 
 // Event-mode launch: one cluster per workgroup, one event per step.
@@ -116,6 +122,15 @@ inline int fixed_dt_lds_bytes(int h_cap, int e_cap, int g_cap)
     return 16 * h_cap + 4 * g_cap + 16 * e_cap;
 }
 
+// Global-memory form: scratch bytes per workgroup (a multiple of 256), and the
+// LDS of the cluster arrays kept there (the per-wave extra is added at launch).
+inline int64_t fixed_dt_global_stride(int h_cap, int e_cap, int g_cap, bool e_lds)
+{
+    const int64_t b = 16 * (int64_t)h_cap + 4 * (int64_t)g_cap + (e_lds ? 0 : 16 * (int64_t)e_cap);
+    return (b + 255) / 256 * 256;
+}
+inline int fixed_dt_global_lds(int e_cap, bool e_lds) { return e_lds ? 16 * e_cap : 0; }
+
 // Cells per axis of a cluster's hole grid: about holes_per_cell holes a cell.
 inline int grid_dim(int n_h, double holes_per_cell)
 {
@@ -127,7 +142,7 @@ inline int grid_dim(int n_h, double holes_per_cell)
 
 // LDS beyond fixed_dt_lds_bytes for a workgroup of `waves` wavefronts: the
 // per-wave exchange slots (2 x waves words) and the persistent ticket.
-inline int fixed_dt_waves_lds(int waves)
+constexpr int fixed_dt_waves_lds(int waves)
 {
     return waves > 1 ? ((4 * (2 * waves + 1) + 15) / 16) * 16 : 0;
 }
@@ -139,6 +154,12 @@ hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int wav
 hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream);
 hipError_t prepare_fixed_dt(int lds_bytes, int max_lds);
 hipError_t fixed_dt_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
+// The global-memory form: persistent launches only (a.queue set, grid at most the
+// slices of a.gscratch); lds_bytes = fixed_dt_global_lds.
+hipError_t launch_fixed_dt_global(const LaunchArgs& a, int grid, int lds_bytes, int waves,
+                                  hipStream_t stream);
+hipError_t prepare_fixed_dt_global(int max_lds);
+hipError_t fixed_dt_global_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
 hipError_t launch_lab(const LabArgs& a, int lds_bytes, hipStream_t stream);
 hipError_t prepare_lab(int lds_bytes);
