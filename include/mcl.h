@@ -107,12 +107,12 @@ const char* mcl_last_error(const void* ctx);
  * stream); NULL restores the context's own stream. */
 int         mcl_set_stream(void* ctx, void* hip_stream);
 /* Largest cluster (electrons, holes) one workgroup can hold in LDS.  Fixed-dt
- * runs (mcl_run_fixed_dt, plans, mcl_run_cluster, mcl_cluster_state) take larger
- * clusters, up to 32768 electrons and 65534 holes, in a global-memory form: holes
- * and cell starts in a scratch slice per resident workgroup (16 B per hole slot,
- * 4 B per cell), electrons in LDS while 16 B each fit; same results, lower
- * speed.  Event-mode and lab runs, and mcl_run_fixed_dt_noalloc, stay within
- * this limit. */
+ * runs (mcl_run_fixed_dt, plans, mcl_run_cluster, mcl_cluster_state) and
+ * event-mode runs take larger clusters, up to 32768 (fixed-dt) or 65535 (event)
+ * electrons and 65534 holes, in a global-memory form: holes and cell starts in a
+ * scratch slice per resident workgroup (16 B per hole slot, 4 B per cell),
+ * electrons in LDS while 16 B each fit; same results, lower speed.  Lab runs and
+ * mcl_run_fixed_dt_noalloc stay within this limit. */
 int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t* max_holes);
 /* Kernel tuning knobs of a context, applied to plans made afterwards.  Keys:
  *   "holes_per_cell"  mean holes per cell of each cluster's nearest-hole grid
@@ -132,9 +132,10 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     waves that cover them), or 0 (default): each launch, or
  *                     each size class of a persistent launch, takes the count
  *                     that keeps the most waves resident.
- *   "global_memory"   1: fixed-dt plans keep every cluster's holes in the
- *                     global-memory form (results identical, slower); 0
- *                     (default): only clusters that do not fit LDS.
+ *   "global_memory"   1: fixed-dt plans and event-mode runs keep every
+ *                     cluster's holes in the global-memory form (results
+ *                     identical, slower); 0 (default): only clusters that do
+ *                     not fit LDS.
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

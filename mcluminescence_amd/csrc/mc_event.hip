@@ -31,6 +31,13 @@
 //   t <= duration and stops after a step that reaches duration (simulate.py:51,91).
 // The oracle (oracle/mcl_oracle.c, orc_run_event) replays the same arithmetic in
 // the same order, so every row is bit-exact.
+//
+// LDS: hole [h_cap] and cstart [g_cap] as in the fixed-dt kernel, then est, ew,
+// the row sums and rq [e_cap each].  Clusters beyond LDS take the fixed-dt
+// kernel's global-memory form (GM): hole and cstart in the workgroup's scratch
+// slice, the electron arrays in LDS while they fit (e_lds) or after cstart in the
+// slice, persistent workgroups over a ticket; the wave's fences widen to
+// workgroup scope so its global stores complete before its lanes read them.
 #include "mc_common.h"
 
 namespace mcl {
@@ -41,6 +48,21 @@ __device__ __forceinline__ double exp1_of(uint32_t u)
     return -log_own(((double)u + 0.5) * 0x1p-32);
 }
 
+// Ordering of one wave's writes before its lanes read them: a wavefront-scope
+// fence for LDS, workgroup scope (the wave's global stores complete) for GM.
+template <bool GM>
+__device__ __forceinline__ void event_fence()
+{
+    if constexpr (GM) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        wave_lds_fence();
+    }
+}
+
+template <bool GM>
 __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int c,
                                                   unsigned char* lds_raw)
 {
@@ -51,10 +73,17 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
     const int n_e0 = uniform(R.n_e0);
     const int h_cap = a.h_cap, e_cap = a.e_cap, g_cap = a.g_cap;
 
-    float4* hole = (float4*)lds_raw;
-    float* hole_f = (float*)lds_raw;
+    unsigned char* hbase = lds_raw;     // hole, cstart
+    unsigned char* ebase = lds_raw + 16 * h_cap + 4 * g_cap;   // est, ew, row sums, rq
+    if constexpr (GM) {
+        unsigned char* const slice = a.gscratch + (size_t)blockIdx.x * (size_t)a.gstride;
+        hbase = slice;
+        ebase = a.e_lds ? lds_raw : slice + 16 * (size_t)h_cap + 4 * (size_t)g_cap;
+    }
+    float4* hole = (float4*)hbase;
+    float* hole_f = (float*)hbase;
     uint32_t* cstart = (uint32_t*)(hole + h_cap);
-    uint32_t* est = cstart + g_cap;
+    uint32_t* est = (uint32_t*)ebase;
     float* ew = (float*)(est + e_cap);
     double* rtot = (double*)(ew + e_cap);            // row sums (the ethr words)
     uint32_t* rq = (uint32_t*)(ew + e_cap) + e_cap;
@@ -120,7 +149,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
         }
-        wave_lds_fence();
+        event_fence<GM>();
         const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
         const double lam = (R.D == 0.0 || n_live + filled == N_int)
                                ? 1e-20
@@ -186,11 +215,11 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             }
             n_live = n_new;
             recomb += 1;
-            wave_lds_fence();
+            event_fence<GM>();
             if (nq > 0) {
                 search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane,
                                 dist_evals);
-                wave_lds_fence();
+                event_fence<GM>();
             }
             ev = 1;
         }
@@ -226,23 +255,56 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
     }
 }
 
+// One workgroup per cluster, or (queue set: the global-memory form) persistent
+// workgroups taking clusters 0 .. n_clusters-1 by ticket until it passes the end.
+template <bool GM>
 __global__ __launch_bounds__(64) void event_kernel(EventArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    if ((int)blockIdx.x < a.n_clusters) run_cluster_event(a, (int)blockIdx.x, lds_raw);
+    if (a.queue == nullptr) {
+        if ((int)blockIdx.x < a.n_clusters) run_cluster_event<GM>(a, (int)blockIdx.x, lds_raw);
+        return;
+    }
+    for (;;) {
+        int i = 0;
+        if (threadIdx.x == 0) i = (int)atomicAdd(a.queue, 1u);
+        i = __shfl(i, 0);
+        if (i >= a.n_clusters) break;
+        run_cluster_event<GM>(a, i, lds_raw);
+        __syncthreads();
+    }
 }
 
 hipError_t prepare_event(int lds_bytes)
 {
-    return hipFuncSetAttribute((const void*)event_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    hipError_t e = hipFuncSetAttribute((const void*)event_kernel<false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)event_kernel<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+    return e;
 }
 
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream)
 {
-    hipLaunchKernelGGL(event_kernel, dim3((unsigned)a.n_clusters), dim3(kWave),
+    hipLaunchKernelGGL(event_kernel<false>, dim3((unsigned)a.n_clusters), dim3(kWave),
                        (unsigned)lds_bytes, stream, a);
     return hipGetLastError();
+}
+
+hipError_t launch_event_global(const EventArgs& a, int grid, int lds_bytes, hipStream_t stream)
+{
+    if (a.queue == nullptr || a.gscratch == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(event_kernel<true>, dim3((unsigned)grid), dim3(kWave), (unsigned)lds_bytes,
+                       stream, a);
+    return hipGetLastError();
+}
+
+hipError_t event_global_occupancy(int lds_bytes, int* blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        (const void*)event_kernel<true>, kWave,
+                                                        (size_t)lds_bytes);
 }
 
 }  // namespace mcl

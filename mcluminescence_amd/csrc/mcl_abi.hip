@@ -1165,10 +1165,18 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     const int h_cap = ((max_h + 3) / 4) * 4 + 4;
     const int e_cap = std::max(64, ((max_e + 63) / 64) * 64);
     const int g_cap = ((max_cells + 1 + 3) / 4) * 4;
-    const int lds = mcl::fixed_dt_lds_bytes(h_cap, e_cap, g_cap);
-    if (lds > c->max_lds)
-        return fail(c, MCL_ERANGE, "cluster of %d electrons / %d holes needs %d B of LDS; a "
-                    "workgroup has %d B", max_e, max_h, lds, c->max_lds);
+    // clusters beyond LDS: holes (and electrons, when they do not fit either) in a
+    // scratch slice per resident workgroup, persistent over a ticket
+    const Layout L = layout_for(c, h_cap, e_cap, g_cap);
+    const int lds = L.lds;
+    int ggrid = 0;
+    if (L.global) {
+        int pc = 0;
+        const hipError_t oe = mcl::event_global_occupancy(lds, &pc);
+        if (oe != hipSuccess)
+            return fail(c, MCL_EDEVICE, "occupancy query: %s", hipGetErrorString(oe));
+        ggrid = (int)std::min<int64_t>(total, (int64_t)std::max(pc, 1) * c->n_cu);
+    }
     std::vector<int32_t> crun((size_t)total), csim((size_t)total);
     {
         int64_t k = 0;
@@ -1205,6 +1213,11 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     if (e == hipSuccess && !dev) e = dmalloc((void**)&d_lum, rowsz);
     if (e == hipSuccess && !dev && t_axis) e = dmalloc((void**)&d_t, sizeof(double) * rowsz);
     if (e == hipSuccess && !dev && e_ratio) e = dmalloc((void**)&d_er, sizeof(double) * rowsz);
+    unsigned char* d_gscratch = nullptr;
+    uint32_t* d_ticket = nullptr;
+    if (e == hipSuccess && L.global)
+        e = dmalloc((void**)&d_gscratch, (size_t)ggrid * (size_t)L.gstride);
+    if (e == hipSuccess && L.global) e = dmalloc((void**)&d_ticket, sizeof(uint32_t));
     if (e != hipSuccess) {
         release(true);
         return fail(c, e == hipErrorOutOfMemory ? MCL_ENOMEM : MCL_EDEVICE, "allocation: %s",
@@ -1218,7 +1231,7 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_csim, csim.data(), sizeof(int32_t) * (size_t)total,
                            hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = mcl::prepare_event(lds);
+    if (e == hipSuccess) e = mcl::prepare_event(std::min(c->max_lds, std::max(lds, 1)));
     mcl::EventArgs a{};
     a.runs = d_runs;
     a.c_run = d_crun;
@@ -1236,11 +1249,18 @@ int mcl_run_event(void* ctx, const mcl_run_params* runs, int32_t n_runs, const d
     a.e_cap = e_cap;
     a.g_cap = g_cap;
     a.wave_search_max = (int32_t)c->wave_search_max;
+    a.queue = d_ticket;
+    a.gscratch = d_gscratch;
+    a.gstride = L.gstride;
+    a.e_lds = L.e_lds;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     if (e == hipSuccess) e = hipEventRecord(ev0, c->stream);
-    if (e == hipSuccess) e = mcl::launch_event(a, lds, c->stream);
+    if (e == hipSuccess && L.global) e = mcl::launch_reset_tickets(d_ticket, 1, c->stream);
+    if (e == hipSuccess)
+        e = L.global ? mcl::launch_event_global(a, ggrid, lds, c->stream)
+                     : mcl::launch_event(a, lds, c->stream);
     if (e == hipSuccess) e = hipEventRecord(ev1, c->stream);
     std::vector<int32_t> hs((size_t)total);
     std::vector<uint64_t> hw(4 * (size_t)total);

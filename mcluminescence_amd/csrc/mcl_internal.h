@@ -90,6 +90,13 @@ struct EventArgs {
     uint64_t* work;         // [n_clusters][4] or null
     int32_t h_cap, e_cap, g_cap;
     int32_t wave_search_max;
+    // global-memory form (clusters beyond LDS), as LaunchArgs: persistent over a
+    // ticket, workgroup b's holes and cell starts (and, unless e_lds, its electron
+    // arrays) at gscratch + b * gstride
+    uint32_t* queue;
+    unsigned char* gscratch;
+    int64_t gstride;
+    int32_t e_lds;
 };
 constexpr int32_t kEventFillUnsupported = -0x7fffffff;
 
@@ -161,6 +168,9 @@ hipError_t launch_fixed_dt_global(const LaunchArgs& a, int grid, int lds_bytes, 
 hipError_t prepare_fixed_dt_global(int max_lds);
 hipError_t fixed_dt_global_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
+// The event kernel's global-memory form: grid workgroups over a.queue's ticket.
+hipError_t launch_event_global(const EventArgs& a, int grid, int lds_bytes, hipStream_t stream);
+hipError_t event_global_occupancy(int lds_bytes, int* blocks_per_cu);
 hipError_t launch_lab(const LabArgs& a, int lds_bytes, hipStream_t stream);
 hipError_t prepare_lab(int lds_bytes);
 hipError_t prepare_event(int lds_bytes);

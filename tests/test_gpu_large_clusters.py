@@ -1,4 +1,5 @@
-"""Clusters beyond one workgroup's LDS: the fixed-dt kernels' global-memory form.
+"""Clusters beyond one workgroup's LDS: the global-memory form of the fixed-dt and
+event-mode kernels.
 
 A cluster whose holes, cell starts and electron arrays need more LDS than a
 workgroup has keeps its holes and cell starts in a global scratch slice (and its
@@ -12,7 +13,7 @@ import numpy as np
 import pytest
 
 from mcluminescence_amd import workloads as W
-from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT, MCL_ERANGE, MclError
+from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT, MCL_ERANGE, MclError, RunParams
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -195,3 +196,37 @@ def test_global_form_uneven_sweep_persistent(global_form):
     # 240 sets of 64-2048 electrons: five size classes, every one in global form
     runs = W.uneven_sweep(n_sets=240, n_steps=192, seed=11)
     _check(global_form, runs, n_bins=48, flags=F_PERSISTENT | F_GRAPH)
+
+
+def _event_run(n, sims):
+    # the glow run in event mode: one event per step, at most 0.1 s a step
+    p = RunParams.make(**{**_tl(n, 1, 1).as_dict(), "sims": sims, "n_steps": 1})
+    p.dt = 0.1
+    return p
+
+
+def _check_event(ctx, run, duration, max_steps):
+    print(f"event mode, large clusters: {orc.geometry(run).n_e0} electrons x {run.sims}",
+          flush=True)
+    lum, t, er, steps, stats = ctx.run_event([run], [duration], max_steps)
+    rl, rt, rer, rsteps = orc.run_event([run], [duration], max_steps, threads=8)
+    np.testing.assert_array_equal(steps, rsteps)
+    np.testing.assert_array_equal(lum, rl)
+    np.testing.assert_array_equal(t, rt)
+    np.testing.assert_array_equal(er, rer)
+    assert stats.recombinations == int(lum.sum())
+
+
+@pytest.mark.parametrize("n,sims", [(5000, 3), (16000, 2)])
+def test_event_mode_beyond_lds(gpu_ctx, n, sims):
+    # 5000: holes in global memory, electrons in LDS; 16000: both in global memory
+    run = _event_run(n, sims)
+    assert not _lds_fits(gpu_ctx, run)
+    _check_event(gpu_ctx, run, 400.0, 4000 + n + 200)
+
+
+def test_event_mode_global_form_small(global_form):
+    _check_event(global_form, _event_run(500, 4), 400.0, 5000)
+    iso = RunParams.make(**{**W.iso_decay_4096x400(sims=1)[0].as_dict(), "sims": 16,
+                            "n_steps": 1})
+    _check_event(global_form, iso, 2000.0, 500)
