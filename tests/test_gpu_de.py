@@ -92,7 +92,22 @@ def test_search_is_deterministic_and_converges(net):
 
 @pytest.mark.parametrize("pop", [300, 1024])
 def test_fused_generation_matches_layered(net, pop):
+    # the surrogate's own shapes: the fused kernel's unrolled form
     print(f"DE: fused vs layered generations, 40 generations (pop {pop})", flush=True)
+    _fused_vs_layered(net, pop)
+
+
+def test_fused_generation_matches_layered_other_shapes(gpu_ctx):
+    # 6 -> 128 x 2 -> 128: the fused kernel's form for any shape it takes
+    print("DE: fused vs layered generations, 6 -> 128 x 2 -> 128, pop 300", flush=True)
+    small = S.Network(gpu_ctx, n_hidden=128, n_hidden_layers=2, n_out=128, max_batch=4096, seed=3)
+    try:
+        _fused_vs_layered(small, 300)
+    finally:
+        small.close()
+
+
+def _fused_vs_layered(net, pop):
     import torch
     tgt = _target(net, [-0.1, 0.25, 0.6, -0.45, 0.3]).cpu().numpy()
     res = []
