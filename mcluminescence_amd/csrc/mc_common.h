@@ -322,6 +322,29 @@ __device__ __forceinline__ float w_of(uint32_t d2bits, float alpha_f)
     return exp_own(-(alpha_f * __builtin_sqrtf(__uint_as_float(d2bits))));
 }
 
+// What a cluster keeps in an electron's w slot: w itself, or (fixed) the fire
+// threshold of x = fma(bdt, w, xA) as its 32-bit pattern, 0xFFFFFFFF for "always"
+// (no other threshold reaches it: p < 1 in float32 gives at most 2^32 - 256), for
+// runs in which that threshold is constant while w is (isothermal, one
+// localisation channel).  The step then reads the threshold instead of taking
+// exp of every electron's x again.
+struct WSlot {
+    bool fixed;
+    float bdt, xA;
+    __device__ __forceinline__ float of(float w) const
+    {
+        if (!fixed) return w;
+        const FireThr t = fire_threshold(__builtin_fmaf(bdt, w, xA));
+        return __uint_as_float(t.always ? 0xFFFFFFFFu : t.thr);
+    }
+};
+
+__device__ __forceinline__ FireThr slot_threshold(float v)
+{
+    const uint32_t b = __float_as_uint(v);
+    return FireThr{b, b == 0xFFFFFFFFu};
+}
+
 __device__ __forceinline__ void nn_from_key(uint64_t key, float alpha_f, uint32_t& slot, float& w)
 {
     const uint32_t d2bits = (uint32_t)(key >> 32);
@@ -361,7 +384,8 @@ __device__ __forceinline__ Grid setup_cluster_t(const RunDesc& R, const Electron
                                                 float4* hole, uint32_t* cstart, uint32_t* est,
                                                 float* ew, uint32_t prun, uint32_t psim,
                                                 uint32_t k0, uint32_t k1, int tid,
-                                                uint64_t& dist_evals)
+                                                uint64_t& dist_evals,
+                                                const WSlot& ws = WSlot{false, 0.0f, 0.0f})
 {
     constexpr int NT = NW * kWave;
     const int lane = NW == 1 ? tid : (tid & (kWave - 1));
@@ -437,7 +461,7 @@ __device__ __forceinline__ Grid setup_cluster_t(const RunDesc& R, const Electron
             float w;
             nn_from_key(key, alpha_f, slot, w);
             est[e] = (slot << 16) | (uint32_t)e;
-            ew[e] = w;
+            ew[e] = ws.of(w);
         }
     }
     __syncthreads();
@@ -457,11 +481,12 @@ __device__ __forceinline__ Grid setup_cluster(const RunDesc& R, const ElectronGe
 // New nearest holes for the queued electrons rq[0..nq) ((slot << 16) | index):
 // wave-cooperative searches, kSearchBatch electrons at a time, while the queue
 // is short (at most wave_search_max), lane-parallel otherwise; writes est and ew
-// of their slots.
+// of their slots (ew as ws keeps it).
 __device__ __forceinline__ void search_queue_nn(const Grid& grid, const ElectronGen& egen,
                                                 uint32_t* est, float* ew, const uint32_t* rq,
                                                 int nq, int wave_search_max, float alpha_f,
-                                                int lane, uint64_t& dist_evals)
+                                                int lane, uint64_t& dist_evals,
+                                                const WSlot& ws = WSlot{false, 0.0f, 0.0f})
 {
     if (nq <= wave_search_max) {
         for (int i0 = 0; i0 < nq; i0 += kSearchBatch) {
@@ -479,7 +504,7 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
                     float w;
                     nn_from_key(key, alpha_f, slot, w);
                     est[qq >> 16] = (slot << 16) | (qq & 0xFFFFu);
-                    ew[qq >> 16] = w;
+                    ew[qq >> 16] = ws.of(w);
                 }
                 continue;
             }
@@ -510,7 +535,7 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
                 float w;
                 nn_from_key(key, alpha_f, slot, w);
                 est[qq >> 16] = (slot << 16) | (qq & 0xFFFFu);
-                ew[qq >> 16] = w;
+                ew[qq >> 16] = ws.of(w);
             }
         }
         return;
@@ -528,7 +553,7 @@ __device__ __forceinline__ void search_queue_nn(const Grid& grid, const Electron
             float w;
             nn_from_key(key, alpha_f, slot, w);
             est[q >> 16] = (slot << 16) | o;
-            ew[q >> 16] = w;
+            ew[q >> 16] = ws.of(w);
         }
     }
 }

@@ -169,22 +169,48 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     const uint32_t psim = (uint32_t)sim << 4;
     const ElectronGen egen{prun, psim | MCL_TAG_GEOM_E, k0, k1, R.side_nm};
 
+    const bool two_ch = R.two_channel != 0;
+    const bool always2 = R.always_chan2 != 0;
+    // single-trap parameter sets share one Boltzmann factor between the channels
+    const bool b1_is_a = R.b == R.s && R.E_loc_1 == R.E_cb;
+    const bool b2_is_b1 = R.E_loc_2 == R.E_loc_1;
+    // Step scalars of step m: start-of-step temperature, Boltzmann factors in
+    // float64, their float32 products with dt, the threshold of the band term.
+    struct StepScalars {
+        float xA, b1dt, b2dt;
+        FireThr tA;
+    };
+    auto scalars_at = [&](int m) {
+        const double t = (double)m * R.dt;
+        const double T = R.T0_K + R.T_rate * t;
+        const double kT = R.k_b * T;
+        const double A = R.s * exp_own(-(R.E_cb / kT));
+        const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
+        const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
+        StepScalars sc;
+        sc.xA = (float)(A * R.dt);
+        sc.b1dt = (float)(B1 * R.dt);
+        sc.b2dt = (float)(B2 * R.dt);
+        sc.tA = fire_threshold(sc.xA);
+        return sc;
+    };
+    const StepScalars sc0 = scalars_at(0);
+    // An isothermal run with one localisation channel keeps each electron's fire
+    // threshold, constant while its w is, in the w slot (state dumps keep w).
+    const bool fixedw = R.T_rate == 0.0 && !two_ch && a.dump.enabled == 0;
+    const WSlot wslot{fixedw, always2 ? sc0.b2dt : sc0.b1dt, sc0.xA};
+
     uint64_t dist_evals = 0;
     const Grid grid = setup_cluster_t<NW>(R, egen, hole, cstart, est, ew, prun, psim, k0, k1, tid,
-                                          dist_evals);
+                                          dist_evals, wslot);
     const float alpha_f = (float)R.alpha_nm;
 
     // ---- step loop (simulate.py:51-92) -------------------------------------------
     const int nb = a.n_bins > 0 ? a.n_bins : n_steps;   // n_bins == 0: one bin per step
     uint32_t* cnt = a.counts + (size_t)c * (size_t)a.row_len;
-    const bool two_ch = R.two_channel != 0;
-    const bool always2 = R.always_chan2 != 0;
     const uint64_t ret_thr = R.ret_thr;
     const uint32_t c3f = psim | MCL_TAG_FIRE, c3c = psim | MCL_TAG_CHAN;
     const int step_limit = a.dump.enabled ? min(a.dump.max_steps, n_steps) : n_steps;
-    // single-trap parameter sets share one Boltzmann factor between the channels
-    const bool b1_is_a = R.b == R.s && R.E_loc_1 == R.E_cb;
-    const bool b2_is_b1 = R.E_loc_2 == R.E_loc_1;
 
     int n_live = n_e0, n_slots = n_e0;
     uint64_t lsteps = 0;
@@ -202,31 +228,34 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     auto claim = [&](uint32_t nn, uint32_t o) {
         atomicMin(&hole_u[4 * nn + 3], (o << 16) | (hole_u[4 * nn + 3] & 0xFFFFu));
     };
-    // Test of one live electron (slot p) with draw u and x = k dt; candidates claim
-    // their hole and remember their threshold for a conflict retry.
-    auto fire_tail = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
+    // Fire threshold of the electron whose w slot holds v, in channel ch (2 when
+    // set): the one kept in the slot, or that of x = k dt = fma(b dt, w, x_A).
+    auto slot_thr = [&](float v, bool ch) -> FireThr {
+        if (fixedw) return slot_threshold(v);
+        const float x = __builtin_fmaf(ch ? b2dt : b1dt, v, xA);
+        return x != xA ? fire_threshold(x) : tA;
+    };
+    // Test of one live electron (slot p) with draw u and threshold t; candidates
+    // claim their hole and remember their threshold for a conflict retry.
+    auto fire_tail = [&](uint32_t st, uint32_t o, int p, uint32_t u, FireThr t, int rr,
                          uint64_t& cand) {
         const uint32_t nn = st >> 16;
         if (nn == kNoHole) return;
-        FireThr t = tA;
-        if (x != xA) t = fire_threshold(x);
         if (t.always || u < t.thr) {
             claim(nn, o);
             ethr[p] = t.always ? 0xFFFFFFFFu : t.thr;
             cand |= 1ull << rr;
         }
     };
-    // Pass-A test of one live electron (slot p, draw u, x = k dt against its current
+    // Pass-A test of one live electron (slot p, draw u, threshold t of its current
     // w): nothing unless the draw fires; then one 16-byte read of its hole gives
     // both the removed-hole marker and the claim field.  A removed hole queues the
     // electron for a new search (its draw kept in ethr; the test against the old w
     // bounds the one against the new); a live one is claimed.  Returns queued.
-    auto test_fire = [&](uint32_t st, uint32_t o, int p, uint32_t u, float x, int rr,
+    auto test_fire = [&](uint32_t st, uint32_t o, int p, uint32_t u, FireThr t, int rr,
                          uint64_t& cand) -> bool {
         const uint32_t nn = st >> 16;
         if (nn == kNoHole) return false;
-        FireThr t = tA;
-        if (x != xA) t = fire_threshold(x);
         if (!(t.always || u < t.thr)) return false;
         const float4 h = hole[nn];
         if (h.x == kDeadHoleX) {
@@ -241,7 +270,8 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     // Search again for the queued electrons q[0..nq): one electron at a time
     // with the whole wave while the queue is short, lane-parallel otherwise.
     auto search_queue = [&](const uint32_t* q, int nq) {
-        search_queue_nn(grid, egen, est, ew, q, nq, a.wave_search_max, alpha_f, lane, dist_evals);
+        search_queue_nn(grid, egen, est, ew, q, nq, a.wave_search_max, alpha_f, lane, dist_evals,
+                        wslot);
     };
     // Per-wave values between the waves (NW > 1): lane 0 of each wave publishes
     // v, one workgroup barrier, then every wave reads all NW values.  Consecutive
@@ -267,26 +297,6 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         }
     };
 
-    // Step scalars of step m: start-of-step temperature, Boltzmann factors in
-    // float64, their float32 products with dt, the threshold of the band term.
-    struct StepScalars {
-        float xA, b1dt, b2dt;
-        FireThr tA;
-    };
-    auto scalars_at = [&](int m) {
-        const double t = (double)m * R.dt;
-        const double T = R.T0_K + R.T_rate * t;
-        const double kT = R.k_b * T;
-        const double A = R.s * exp_own(-(R.E_cb / kT));
-        const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
-        const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
-        StepScalars sc;
-        sc.xA = (float)(A * R.dt);
-        sc.b1dt = (float)(B1 * R.dt);
-        sc.b2dt = (float)(B2 * R.dt);
-        sc.tA = fire_threshold(sc.xA);
-        return sc;
-    };
     // Look-ahead for few live electrons: lane j tests step n + j for every live
     // electron with exactly the draw and threshold that step would use (a stale
     // electron against its old w, as in pass A).  Returns how many steps from n on
@@ -310,9 +320,13 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             bool ch = always2;
             if (two_ch)
                 ch = (uint64_t)pick_word(philox10(mblk, o, prun, c3c, k0, k1), mword) < ret_thr;
-            const float x = __builtin_fmaf(ch ? sc.b2dt : sc.b1dt, w, sc.xA);
-            FireThr t = sc.tA;
-            if (x != sc.xA) t = fire_threshold(x);
+            FireThr t;
+            if (fixedw) {
+                t = slot_threshold(w);
+            } else {
+                const float x = __builtin_fmaf(ch ? sc.b2dt : sc.b1dt, w, sc.xA);
+                t = x != sc.xA ? fire_threshold(x) : sc.tA;
+            }
             fire = fire || t.always || u < t.thr;
         }
         const uint64_t fm = ballot(fire && lane < len);
@@ -331,13 +345,10 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         }
     };
 
-    {
-        const StepScalars sc = scalars_at(0);
-        xA = sc.xA;
-        b1dt = sc.b1dt;
-        b2dt = sc.b2dt;
-        tA = sc.tA;
-    }
+    xA = sc0.xA;
+    b1dt = sc0.b1dt;
+    b2dt = sc0.b2dt;
+    tA = sc0.tA;
     // Philox blocks of the first 2 kCachePairs rows, kept in registers: steps
     // n .. n+3 share the counter (n >> 2, e, run, sim|tag), so a block computed in
     // one of them serves the other three.  cache_pairs marks the row pairs whose
@@ -418,10 +429,13 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
             // Below x = 2^-32 (in either channel) the threshold is 0: no draw can
             // fire, and a stale electron keeps waiting.  Rows whose lanes all sit
-            // there (early in a heating ramp) skip their draws.
-            if (ballot(__builtin_fmaf(bmax, w0, xA) >= 0x1p-32f ||
-                       __builtin_fmaf(bmax, w1, xA) >= 0x1p-32f) == 0)
-                return;
+            // there (early in a heating ramp) skip their draws.  (A slot that keeps
+            // the threshold holds 0 exactly then.)
+            const bool live0 = fixedw ? __float_as_uint(w0) != 0u
+                                      : __builtin_fmaf(bmax, w0, xA) >= 0x1p-32f;
+            const bool live1 = fixedw ? __float_as_uint(w1) != 0u
+                                      : __builtin_fmaf(bmax, w1, xA) >= 0x1p-32f;
+            if (ballot(live0 || live1) == 0) return;
             uint32_t u0 = 0u, u1 = 0u;
             bool ch0 = always2, ch1 = always2;
             bool hit = false;
@@ -458,10 +472,8 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                     cache_pairs |= 1u << q;
                 }
             }
-            const float x0 = __builtin_fmaf(ch0 ? b2dt : b1dt, w0, xA);
-            const float x1 = __builtin_fmaf(ch1 ? b2dt : b1dt, w1, xA);
-            const bool s0 = test_fire(st0, o0, p0, u0, x0, r, cand);
-            const bool s1 = test_fire(st1, o1, p1, u1, x1, r + 1, cand);
+            const bool s0 = test_fire(st0, o0, p0, u0, slot_thr(w0, ch0), r, cand);
+            const bool s1 = test_fire(st1, o1, p1, u1, slot_thr(w1, ch1), r + 1, cand);
             const uint64_t m0 = ballot(s0);
             if (s0) rq_w[nq + lanes_below(m0)] = ((uint32_t)p0 << 16) | o0;
             nq += __popcll(m0);
@@ -489,8 +501,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                     // the draw and channel of pass A, the new w
                     const uint32_t st = est[p];
                     const bool ch = (stale_ch >> r) & 1ull;
-                    fire_tail(st, st & 0xFFFFu, p, ethr[p], __builtin_fmaf(ch ? b2dt : b1dt, ew[p], xA),
-                              r, cand);
+                    fire_tail(st, st & 0xFFFFu, p, ethr[p], slot_thr(ew[p], ch), r, cand);
                 }
             }
         }
@@ -594,9 +605,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                 const u32x4 rc = philox10(blk, o, prun, c3c, k0, k1);
                                 chan2 = (uint64_t)pick_word(rc, word) < ret_thr;
                             }
-                            const float x = __builtin_fmaf(chan2 ? b2dt : b1dt, ew[p], xA);
-                            FireThr t = tA;
-                            if (x != xA) t = fire_threshold(x);
+                            const FireThr t = slot_thr(ew[p], chan2);
                             const uint64_t Tn = t.always ? 0x100000000ull : (uint64_t)t.thr;
                             const uint32_t eo = ethr[p];
                             const uint64_t To = eo == 0xFFFFFFFFu ? 0x100000000ull : (uint64_t)eo;
