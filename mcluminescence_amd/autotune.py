@@ -35,8 +35,8 @@ from ._abi import DEFAULT_SEED, F_GRAPH, F_PERSISTENT, Context, RunParams, geome
 TABLE = Path(__file__).resolve().parent / "tuning" / "table.json"
 DEFAULTS = {"holes_per_cell": 2.0, "persist_frac": 1.0, "wave_search_max": 32.0,
             "lookahead_max": 8.0, "waves_per_cluster": 0.0}
-SPACE = {"holes_per_cell": [2.0, 3.0, 4.0], "persist_frac": [1.0, 2.0],
-         "wave_search_max": [8.0, 32.0, 64.0], "lookahead_max": [0.0, 8.0, 32.0],
+SPACE = {"holes_per_cell": [2.0, 2.5, 3.0, 4.0], "persist_frac": [1.0, 2.0],
+         "wave_search_max": [8.0, 32.0, 64.0], "lookahead_max": [0.0, 4.0, 8.0, 32.0],
          "waves_per_cluster": [0.0, 1.0]}
 
 
