@@ -307,6 +307,44 @@ def sweep_extras(ctx) -> dict:
     return out
 
 
+def large_cluster_extras(ctx) -> dict:
+    """Clusters beyond one workgroup's LDS: TL glow curves (E = 1.3 eV, s = 1e12 1/s,
+    alpha = 4e9 1/m, rho = 1e23 m^-3, 300 -> 700 K at 1 K/s, dt 0.1 s, 4000 steps) at
+    about 2 M electrons per launch, from 3,000 electrons a cluster (LDS) to 32,768
+    (holes, cell starts and electrons in a global scratch slice per resident
+    workgroup).  Device time of a launch from HIP events around it (median of 3)."""
+    import numpy as np
+    import torch
+    from mcluminescence_amd import workloads as W
+    out = {"workload": "TL glow 300 -> 700 K, 4000 steps of 0.1 s, about 2 M electrons per "
+                       "launch", "cases": []}
+    for n in (3000, 4000, 8000, 16000, 32768):
+        sims = max(8, 2_000_000 // n)
+        runs = [W.single_trap(1.3, 1e12, 4e9, 1e23, n, T_start_K=300.0, T_rate=1.0, dt=0.1,
+                              n_steps=4000, sims=sims)]
+        plan = ctx.plan(runs)
+        info = plan.info
+        counts = torch.zeros((info.total_clusters, info.row_len), dtype=torch.int32,
+                             device="cuda")
+        live = torch.zeros(info.total_clusters, dtype=torch.int64, device="cuda")
+        plan.launch(counts.data_ptr(), live.data_ptr())
+        plan.sync()
+        ms = []
+        for _ in range(3):
+            plan.launch(counts.data_ptr(), live.data_ptr())
+            plan.sync()
+            ms.append(plan.last_ms())
+        w = plan.work()
+        m = float(np.median(ms))
+        out["cases"].append({"electrons": n, "clusters": sims, "ms": m,
+                             "live_electron_steps_per_s": w.live_electron_steps / (m * 1e-3),
+                             "lds_bytes": int(info.lds_bytes), "grid": int(info.grid),
+                             "block": int(info.block)})
+        print(f"large clusters: {n} electrons x {sims}: {m:.2f} ms", file=sys.stderr, flush=True)
+        plan.close()
+    return out
+
+
 def cpu_harness(seconds: float) -> dict:
     """The C replay harness (the fixed-dt algorithm the kernels run, OpenMP over
     clusters) on bounded samples: the decay clusters on one core and on every host
@@ -480,6 +518,9 @@ def main() -> int:
     ap.add_argument("--sweep-only", action="store_true",
                     help="only time the uneven 10,000-set sweep (and its CPU harness leg "
                          "unless --no-cpu)")
+    ap.add_argument("--large-clusters", action="store_true",
+                    help="only time TL glow launches of clusters from 3,000 to 32,768 "
+                         "electrons (the global-memory form beyond LDS)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB three times (the "
                          "reference point for hardware byte counters) and exit")
@@ -498,6 +539,12 @@ def main() -> int:
         if not args.no_cpu:
             res["cpu_harness"] = cpu_harness(args.cpu_seconds)
         print(json.dumps(res), flush=True)
+        return 0
+    if args.large_clusters:
+        from mcluminescence_amd._abi import Context
+        ctx = Context(0)
+        print(json.dumps(large_cluster_extras(ctx)), flush=True)
+        ctx.close()
         return 0
     if args.copy_gib > 0:
         import torch
@@ -580,7 +627,7 @@ def main() -> int:
         "config": {"workload": "iso_decay_500K_4096x400", "clusters": n, "electrons": 400,
                    "holes": 691, "T_K": 500.0, "dt_s": 0.5, "n_steps": 4000, "E_eV": 1.4,
                    "s_per_s": 1e13, "alpha_per_m": 4e9, "rho_per_m3": 1e23,
-                   "kernel": "fixed_dt_kernel<false> (1 wave64 per cluster, clusters in LDS)", "knobs": knobs},
+                   "kernel": "fixed_dt_kernel<false, true> (1 wave64 per cluster, clusters in LDS, fire thresholds kept in the w slots)", "knobs": knobs},
         "work_per_step": {"live_electron_steps": w.live_electron_steps,
                           "distance_evals": w.distance_evals,
                           "recombinations": w.recombinations,

@@ -124,8 +124,11 @@ __device__ __forceinline__ void queue_sync()
 // state in lds_raw (GM: holes and cell starts, and the electron arrays unless
 // a.e_lds, in the workgroup's scratch slice).  Every branch that holds a
 // workgroup barrier depends only on values all waves share (n, n_live, n_slots,
-// and results of the exchanges).
-template <int NW, bool GM>
+// and results of the exchanges).  FW: the build with the fixed-threshold path
+// (isothermal one-channel runs keep fire thresholds in their w slots); without
+// it every run computes its thresholds from w, and the code of that path is not
+// in the kernel.  Both builds give the same counts for every run.
+template <int NW, bool GM, bool FW>
 __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                             unsigned char* lds_raw)
 {
@@ -197,7 +200,7 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     const StepScalars sc0 = scalars_at(0);
     // An isothermal run with one localisation channel keeps each electron's fire
     // threshold, constant while its w is, in the w slot (state dumps keep w).
-    const bool fixedw = R.T_rate == 0.0 && !two_ch && a.dump.enabled == 0;
+    const bool fixedw = FW && R.T_rate == 0.0 && !two_ch && a.dump.enabled == 0;
     const WSlot wslot{fixedw, always2 ? sc0.b2dt : sc0.b1dt, sc0.xA};
 
     uint64_t dist_evals = 0;
@@ -784,13 +787,13 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
 // from order[] (null: 0 .. n_order-1) through a global atomic ticket until the
 // list is exhausted.  Every workgroup leaves the loop when the ticket passes
 // n_order.  The global-memory form always runs persistent.
-template <bool GM>
+template <bool GM, bool FW>
 __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (a.queue == nullptr) {
         const int i = blockIdx.x;
-        if (i < a.n_order) run_cluster<1, GM>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        if (i < a.n_order) run_cluster<1, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         return;
     }
     for (;;) {
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
         if (threadIdx.x == 0) i = (int)atomicAdd(a.queue, 1u);
         i = __shfl(i, 0);
         if (i >= a.n_order) break;
-        run_cluster<1, GM>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        run_cluster<1, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         __syncthreads();
     }
 }
@@ -807,13 +810,13 @@ __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
 // most 256 VGPRs a wave, as the one-wave kernel).  The persistent ticket goes
 // through LDS (the word after the exchange slots), read by every wave between
 // two barriers, so all waves take the same cluster and leave the loop together.
-template <int NW, bool GM>
+template <int NW, bool GM, bool FW>
 __global__ __launch_bounds__(NW * 64, 2) void fixed_dt_mw_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (a.queue == nullptr) {
         const int i = blockIdx.x;
-        if (i < a.n_order) run_cluster<NW, GM>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        if (i < a.n_order) run_cluster<NW, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         return;
     }
     uint32_t* const ticket =
@@ -825,18 +828,37 @@ __global__ __launch_bounds__(NW * 64, 2) void fixed_dt_mw_kernel(LaunchArgs a)
         const int i = (int)uniform(*ticket);
         __syncthreads();
         if (i >= a.n_order) break;
-        run_cluster<NW, GM>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        run_cluster<NW, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         __syncthreads();
     }
 }
 
-template <bool GM>
-const void* fixed_dt_fn(int waves)
+template <bool GM, bool FW>
+const void* fixed_dt_fn_t(int waves)
 {
-    return waves == 8 ? (const void*)fixed_dt_mw_kernel<8, GM>
-         : waves == 4 ? (const void*)fixed_dt_mw_kernel<4, GM>
-         : waves == 2 ? (const void*)fixed_dt_mw_kernel<2, GM>
-                      : (const void*)fixed_dt_kernel<GM>;
+    return waves == 8 ? (const void*)fixed_dt_mw_kernel<8, GM, FW>
+         : waves == 4 ? (const void*)fixed_dt_mw_kernel<4, GM, FW>
+         : waves == 2 ? (const void*)fixed_dt_mw_kernel<2, GM, FW>
+                      : (const void*)fixed_dt_kernel<GM, FW>;
+}
+
+template <bool GM>
+const void* fixed_dt_fn(int waves, bool fw)
+{
+    return fw ? fixed_dt_fn_t<GM, true>(waves) : fixed_dt_fn_t<GM, false>(waves);
+}
+
+template <bool GM, bool FW>
+void launch_fixed_dt_w(const LaunchArgs& a, dim3 g, unsigned lds, int waves, hipStream_t stream)
+{
+    if (waves == 8)
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<8, GM, FW>), g, dim3(8 * kWave), lds, stream, a);
+    else if (waves == 4)
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<4, GM, FW>), g, dim3(4 * kWave), lds, stream, a);
+    else if (waves == 2)
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<2, GM, FW>), g, dim3(2 * kWave), lds, stream, a);
+    else
+        hipLaunchKernelGGL((fixed_dt_kernel<GM, FW>), g, dim3(kWave), lds, stream, a);
 }
 
 template <bool GM>
@@ -844,15 +866,26 @@ hipError_t launch_fixed_dt_t(const LaunchArgs& a, int grid, unsigned lds, int wa
                              hipStream_t stream)
 {
     const dim3 g((unsigned)grid);
-    if (waves == 8)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<8, GM>), g, dim3(8 * kWave), lds, stream, a);
-    else if (waves == 4)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<4, GM>), g, dim3(4 * kWave), lds, stream, a);
-    else if (waves == 2)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<2, GM>), g, dim3(2 * kWave), lds, stream, a);
+    if (a.fixed_thr != 0)
+        launch_fixed_dt_w<GM, true>(a, g, lds, waves, stream);
     else
-        hipLaunchKernelGGL((fixed_dt_kernel<GM>), g, dim3(kWave), lds, stream, a);
+        launch_fixed_dt_w<GM, false>(a, g, lds, waves, stream);
     return hipGetLastError();
+}
+
+// Resident workgroups per CU of the NW-wave kernel with this much dynamic LDS:
+// the smaller count of its two builds (the launch may take either).
+template <bool GM>
+hipError_t fixed_dt_occupancy_t(size_t lds, int waves, int* blocks_per_cu)
+{
+    int b0 = 0, b1 = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, fixed_dt_fn<GM>(waves, false),
+                                                                waves * kWave, lds);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, fixed_dt_fn<GM>(waves, true),
+                                                         waves * kWave, lds);
+    *blocks_per_cu = b0 < b1 ? b0 : b1;
+    return e;
 }
 
 }  // namespace mcl

@@ -18,9 +18,12 @@ hipError_t prepare_fixed_dt(int lds_bytes, int max_lds)
 {
     hipError_t e = hipSuccess;
     for (int w : {1, 2, 4, 8}) {
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(fixed_dt_fn<false>(w), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    std::min(lds_bytes + fixed_dt_waves_lds(w), max_lds));
+        for (bool fw : {false, true}) {
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(fixed_dt_fn<false>(w, fw),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        std::min(lds_bytes + fixed_dt_waves_lds(w), max_lds));
+        }
     }
     return e;
 }
@@ -41,9 +44,8 @@ hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream)
 
 hipError_t fixed_dt_occupancy(int lds_bytes, int waves, int* blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, fixed_dt_fn<false>(waves), waves * kWave,
-        (size_t)(lds_bytes + fixed_dt_waves_lds(waves)));
+    return fixed_dt_occupancy_t<false>((size_t)(lds_bytes + fixed_dt_waves_lds(waves)), waves,
+                                       blocks_per_cu);
 }
 
 }  // namespace mcl

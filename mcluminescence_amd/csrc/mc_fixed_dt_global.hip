@@ -11,9 +11,11 @@ hipError_t prepare_fixed_dt_global(int max_lds)
 {
     hipError_t e = hipSuccess;
     for (int w : {1, 2, 4, 8}) {
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(fixed_dt_fn<true>(w), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    max_lds);
+        for (bool fw : {false, true}) {
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(fixed_dt_fn<true>(w, fw),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        }
     }
     return e;
 }
@@ -28,9 +30,8 @@ hipError_t launch_fixed_dt_global(const LaunchArgs& a, int grid, int lds_bytes, 
 
 hipError_t fixed_dt_global_occupancy(int lds_bytes, int waves, int* blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, fixed_dt_fn<true>(waves), waves * kWave,
-        (size_t)(lds_bytes + fixed_dt_waves_lds(waves)));
+    return fixed_dt_occupancy_t<true>((size_t)(lds_bytes + fixed_dt_waves_lds(waves)), waves,
+                                      blocks_per_cu);
 }
 
 }  // namespace mcl

@@ -142,6 +142,7 @@ struct Plan {
     int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0;
     int32_t max_e = 0, max_h = 0;
     int32_t waves = 1;            // wavefronts per cluster of the one-workgroup-per-cluster launch
+    int32_t fixed_thr = 0;        // some run keeps its fire thresholds (keeps_thresholds)
     uint32_t flags = 0;
     // clusters beyond one workgroup's LDS (global-memory form): the plain launch is
     // persistent, ggrid workgroups with a scratch slice of gstride bytes each
@@ -160,6 +161,7 @@ struct Plan {
         int32_t h_cap = 0, e_cap = 0, g_cap = 0, lds = 0, grid = 0;
         int32_t max_e = 0, max_h = 0;
         int32_t waves = 1;
+        int32_t fixed_thr = 0;          // some cluster's run keeps its fire thresholds
         bool global = false;            // global-memory form: scratch at goff, gstride a slice
         int32_t e_lds = 0;
         int64_t gstride = 0;
@@ -241,6 +243,11 @@ int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geo
                     g.n_e0, g.n_h);
     return MCL_OK;
 }
+
+// Runs whose clusters keep their fire thresholds in the w slots (isothermal, one
+// localisation channel): a launch with any of them takes the kernels built with
+// that path.
+bool keeps_thresholds(const mcl::RunDesc& d) { return d.T_rate == 0.0 && !d.two_channel; }
 
 mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id,
                        double holes_per_cell)
@@ -418,6 +425,7 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
             cl.max_e = std::max(cl.max_e, d.n_e0);
             cl.max_h = std::max(cl.max_h, d.n_h);
             max_cells = std::max(max_cells, d.grid * d.grid * d.grid);
+            if (keeps_thresholds(d)) cl.fixed_thr = 1;
         }
         cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
         cl.e_cap = std::max(64, ((cl.max_e + 63) / 64) * 64);
@@ -518,6 +526,8 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->max_e = max_e;
     p->max_h = max_h;
     p->flags = flags;
+    for (int r = 0; r < n_runs; ++r)
+        if (runs[r].sims > 0 && keeps_thresholds(desc[(size_t)r])) p->fixed_thr = 1;
     p->global = L.global;
     p->e_lds = L.e_lds;
     p->gstride = L.gstride;
@@ -604,6 +614,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.wave_search_max = (int32_t)p->ctx->wave_search_max;
     a.lookahead_max = (int32_t)p->ctx->lookahead_max;
     a.dump.enabled = 0;
+    a.fixed_thr = p->fixed_thr;
     return a;
 }
 
@@ -622,6 +633,7 @@ hipError_t enqueue_persistent(Plan* p, uint32_t* counts, uint64_t* live, hipStre
         a.order = p->d_order + cl.first;
         a.n_order = cl.count;
         a.queue = p->d_tickets + k;
+        a.fixed_thr = cl.fixed_thr;
         a.h_cap = cl.h_cap;
         a.e_cap = cl.e_cap;
         a.g_cap = cl.g_cap;
@@ -1425,6 +1437,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     // descriptors and cluster map through a fixed host staging block, one chunk
     // at a time (each copied and waited for before the block is refilled)
     alignas(64) unsigned char stage[16384];
+    int32_t fixed_thr = 0;
     auto upload = [&](size_t dst_off, size_t bytes) -> hipError_t {
         hipError_t r = hipMemcpyAsync(base + dst_off, stage, bytes, hipMemcpyHostToDevice, c->stream);
         if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
@@ -1439,6 +1452,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
                 mcl_geometry g;
                 geometry(runs[r0 + j], g);
                 d[j] = make_desc(runs[r0 + j], g, r0 + j, c->holes_per_cell);
+                if (runs[r0 + j].sims > 0 && keeps_thresholds(d[j])) fixed_thr = 1;
             }
             e = upload(sizeof(mcl::RunDesc) * (size_t)r0, sizeof(mcl::RunDesc) * (size_t)k);
         }
@@ -1492,6 +1506,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     a.wave_search_max = (int32_t)c->wave_search_max;
     a.lookahead_max = (int32_t)c->lookahead_max;
     a.dump.enabled = 0;
+    a.fixed_thr = fixed_thr;
     e = hipEventRecord(c->na_ev0, c->stream);
     if (e == hipSuccess) e = mcl::launch_fixed_dt(a, (int)total, lds, waves, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->na_ev1, c->stream);

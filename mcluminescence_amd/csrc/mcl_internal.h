@@ -72,6 +72,10 @@ struct LaunchArgs {
     unsigned char* gscratch;
     int64_t gstride;
     int32_t e_lds;            // electron arrays in LDS after the exchange slots
+    // 1: some run of the launch is isothermal with one localisation channel, and
+    // the kernels built with the fixed-threshold path run it; 0: the kernels
+    // without that path (same counts, shorter code for ramped or two-channel runs)
+    int32_t fixed_thr;
 };
 
 // Event-mode launch: one cluster per workgroup, one event per step.
