@@ -40,9 +40,9 @@ PEAK_VALU_LANE_SLOTS = 256 * 64 * 2.4e9
 PEAK_HBM_GBPS = 8000.0
 SLOTS_PER_LIVE_ESTEP = 28.0     # 1/4 x (20 v_mad_u64_u32 x 4 + 20 v_xor3) + fma/cmp/select
 SLOTS_PER_DISTANCE = 9.0        # 3 sub, mul, 2 fma, cmp, 2 cndmask
-# HBM bytes per launch of the decay workload, from rocprofv3 FETCH_SIZE (628.4 KiB,
-# x2 by a plain-copy calibration) and WRITE_SIZE (64,260 KiB, x1)
-TRAFFIC_BYTES_DECAY = (628.4 * 2 + 64260.0) * 1024
+# HBM bytes per launch of the decay workload, from rocprofv3 FETCH_SIZE (616.0 KiB,
+# x2 by a plain-copy calibration) and WRITE_SIZE (64,260.6 KiB, x1)
+TRAFFIC_BYTES_DECAY = (616.0 * 2 + 64260.6) * 1024
 
 
 def cpu_baseline(seconds: float) -> dict:
