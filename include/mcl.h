@@ -136,6 +136,12 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     cluster's holes in the global-memory form (results
  *                     identical, slower); 0 (default): only clusters that do
  *                     not fit LDS.
+ *   "fixed_thresholds" 1 (default): a fixed-dt launch (or size class) with an
+ *                     isothermal one-localisation-channel run takes the kernels
+ *                     that keep each electron's fire threshold in its w slot;
+ *                     0: the kernels that compute it from w every step (results
+ *                     identical; ramped and two-channel launches always take
+ *                     those).
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

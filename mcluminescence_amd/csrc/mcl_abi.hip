@@ -247,7 +247,10 @@ int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geo
 // Runs whose clusters keep their fire thresholds in the w slots (isothermal, one
 // localisation channel): a launch with any of them takes the kernels built with
 // that path.
-bool keeps_thresholds(const mcl::RunDesc& d) { return d.T_rate == 0.0 && !d.two_channel; }
+bool keeps_thresholds(const Ctx* c, const mcl::RunDesc& d)
+{
+    return c->fixed_thresholds != 0.0 && d.T_rate == 0.0 && !d.two_channel;
+}
 
 mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id,
                        double holes_per_cell)
@@ -425,7 +428,7 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
             cl.max_e = std::max(cl.max_e, d.n_e0);
             cl.max_h = std::max(cl.max_h, d.n_h);
             max_cells = std::max(max_cells, d.grid * d.grid * d.grid);
-            if (keeps_thresholds(d)) cl.fixed_thr = 1;
+            if (keeps_thresholds(c, d)) cl.fixed_thr = 1;
         }
         cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
         cl.e_cap = std::max(64, ((cl.max_e + 63) / 64) * 64);
@@ -527,7 +530,7 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->max_h = max_h;
     p->flags = flags;
     for (int r = 0; r < n_runs; ++r)
-        if (runs[r].sims > 0 && keeps_thresholds(desc[(size_t)r])) p->fixed_thr = 1;
+        if (runs[r].sims > 0 && keeps_thresholds(c, desc[(size_t)r])) p->fixed_thr = 1;
     p->global = L.global;
     p->e_lds = L.e_lds;
     p->gstride = L.gstride;
@@ -834,6 +837,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->global_memory = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "fixed_thresholds") == 0) {
+        if (!(value == 0.0 || value == 1.0))
+            return fail(c, MCL_EINVAL, "fixed_thresholds must be 0 or 1 (got %g)", value);
+        c->fixed_thresholds = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -867,6 +876,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "global_memory") == 0) {
         *value = c->global_memory;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "fixed_thresholds") == 0) {
+        *value = c->fixed_thresholds;
         return MCL_OK;
     }
     return MCL_EINVAL;
@@ -1452,7 +1465,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
                 mcl_geometry g;
                 geometry(runs[r0 + j], g);
                 d[j] = make_desc(runs[r0 + j], g, r0 + j, c->holes_per_cell);
-                if (runs[r0 + j].sims > 0 && keeps_thresholds(d[j])) fixed_thr = 1;
+                if (runs[r0 + j].sims > 0 && keeps_thresholds(c, d[j])) fixed_thr = 1;
             }
             e = upload(sizeof(mcl::RunDesc) * (size_t)r0, sizeof(mcl::RunDesc) * (size_t)k);
         }

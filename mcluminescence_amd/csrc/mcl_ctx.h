@@ -30,6 +30,8 @@ struct Ctx {
     double mlp_fused = 1.0;        // surrogate inference as one kernel (1) or per layer (0)
     double global_memory = 0.0;    // fixed-dt clusters in the global-memory form: 0 when
                                    // they exceed LDS, 1 always
+    double fixed_thresholds = 1.0; // 1: launches with an isothermal one-channel run take the
+                                   // kernels with the fixed-threshold path; 0: never
     std::string err;
     // timing events of mcl_run_fixed_dt_noalloc, made once by mcl_create
     hipEvent_t na_ev0 = nullptr, na_ev1 = nullptr;

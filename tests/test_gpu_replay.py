@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from mcluminescence_amd import workloads as W
-from mcluminescence_amd._abi import RunParams
+from mcluminescence_amd._abi import F_GRAPH, F_PERSISTENT, RunParams
 from oracle import orc
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +54,26 @@ def test_iso_decay_per_step_bit_exact(gpu_ctx):
     runs = W.iso_decay_4096x400(sims=256)
     g, gl, o, ol, _ = _replay(gpu_ctx, runs)
     _assert_same(g, gl, o, ol)
+
+
+@pytest.mark.parametrize("fixed_thresholds", [0, 1])
+def test_iso_decay_both_kernel_builds_bit_exact(gpu_ctx, fixed_thresholds):
+    # the isothermal decay on the build that keeps each electron's fire threshold in
+    # its w slot (1) and on the one that computes it from w every step (0); also a
+    # mixed launch (a ramped run beside the isothermal one) and the persistent form
+    runs = W.iso_decay_4096x400(sims=128) + W.tl_glow_16x500(sims=16)
+    gpu_ctx.tune_set("fixed_thresholds", fixed_thresholds)
+    try:
+        print(f"fixed_thresholds {fixed_thresholds}", flush=True)
+        g, gl, o, ol, _ = _replay(gpu_ctx, runs[:1])
+        _assert_same(g, gl, o, ol)
+        g, gl, o, ol, _ = _replay(gpu_ctx, runs)
+        _assert_same(g, gl, o, ol)
+        gp, glp, _ = gpu_ctx.run_fixed_dt(runs, seed=0x4D434C554D000001,
+                                          flags=F_PERSISTENT | F_GRAPH)
+        _assert_same(gp, glp, o, ol)
+    finally:
+        gpu_ctx.tune_set("fixed_thresholds", 1)
 
 
 def test_tunnelling_per_step_bit_exact(gpu_ctx):
