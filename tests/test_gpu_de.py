@@ -90,7 +90,7 @@ def test_search_is_deterministic_and_converges(net):
     assert fa < 1e-3 * f_start and fa < 1e-4, (fa, f_start)
 
 
-@pytest.mark.parametrize("pop", [300, 1024])
+@pytest.mark.parametrize("pop", [300, 1024, 2000])
 def test_fused_generation_matches_layered(net, pop):
     # the surrogate's own shapes: the fused kernel's unrolled form
     print(f"DE: fused vs layered generations, 40 generations (pop {pop})", flush=True)
