@@ -316,6 +316,21 @@ __device__ __forceinline__ double row_scan_d(double v, int lane)
     return v;
 }
 
+// The value row_scan_d leaves in lane 63 (the total of the 64 lanes), in every
+// lane, without the scan's cross-row pass: the same four in-row steps, then
+// ((t0 + t1) + t2) + t3 from the row totals (lane 63 holds t3 + ((t0 + t1) + t2),
+// the same sum), read to scalar registers, so no LDS permute returns it.
+__device__ __forceinline__ double row_total_d(double v)
+{
+    v = v + dpp_row_shift_d<0x111>(v);
+    v = v + dpp_row_shift_d<0x112>(v);
+    v = v + dpp_row_shift_d<0x114>(v);
+    v = v + dpp_row_shift_d<0x118>(v);
+    const double t0 = lane_value_d(v, 15), t1 = lane_value_d(v, 31), t2 = lane_value_d(v, 47),
+                 t3 = lane_value_d(v, 63);
+    return ((t0 + t1) + t2) + t3;
+}
+
 // w = exp(-alpha r) of a squared distance (every path computes it this one way).
 __device__ __forceinline__ float w_of(uint32_t d2bits, float alpha_f)
 {

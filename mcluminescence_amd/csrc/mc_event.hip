@@ -10,9 +10,10 @@
 //   B_c = b e^{-E_loc_c/kT} in float64 with the library's exp; per live electron
 //   (index order) k_e = fma(B_ch, w_e, A) in float64, the channel drawn as in the
 //   fixed-dt step (Philox (i >> 2, e, run, sim|CHAN), word i & 3).
-//   K = sum of k_e: each row of 64 consecutive live electrons is summed by an
-//   inclusive scan (row_scan_d: steps of 1, 2, 4, 8 within 16-lane rows, then the
-//   totals of the rows before), and the rows are added in order.
+//   K = sum of k_e: each row of 64 consecutive live electrons is summed as the
+//   last lane of an inclusive scan (row_scan_d: steps of 1, 2, 4, 8 within 16-lane
+//   rows, then the totals of the rows before; row_total_d forms the same value),
+//   and the rows are added in order.
 //   One Philox block (i, 0xFFFFFFFF, run, sim|FIRE) gives three words:
 //   E_t = -ln((u0 + 1/2) 2^-32), U_w = u1 2^-32, E_f = -ln((u2 + 1/2) 2^-32).
 //   dt_fill = E_f (1 / lambda), lambda = 1e-20 when D = 0 or n_e = N_e
@@ -46,6 +47,17 @@ namespace mcl {
 __device__ __forceinline__ double exp1_of(uint32_t u)
 {
     return -log_own(((double)u + 0.5) * 0x1p-32);
+}
+
+// pre e^{-E/kT} as the step computes it (the library's exp of the float64
+// quotient), with the quotient and exp skipped where kT 720 < E: there E / kT >
+// 708, where exp_own returns 0, so the factor is 0 either way (pre_ok: pre finite
+// and > 0).  The tunnelling physics' band (E_cb = 1000 eV) is such a term at
+// every temperature.
+__device__ __forceinline__ double boltzmann(double pre, bool pre_ok, double E, double kT)
+{
+    if (pre_ok && kT * 720.0 < E) return 0.0;
+    return pre * exp_own(-(E / kT));
 }
 
 // Ordering of one wave's writes before its lanes read them: a wavefront-scope
@@ -117,6 +129,14 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
     int i = 0;
     int32_t status = 0;
     uint64_t lsteps = 0, recomb = 0;
+    const bool s_ok = R.s > 0.0 && R.s < __builtin_inf();
+    const bool b_ok = R.b > 0.0 && R.b < __builtin_inf();
+    // Without dose, lambda is 1e-20 and dt_fill >= 1e20 E_f >= 1.1e10 s (E_f >=
+    // -ln(1 - 2^-33)): with dt_cap below that it is never the minimum, so the
+    // step does not draw it (dt_fill = +inf gives the same dt and the same event).
+    const bool fill_never = R.D == 0.0 && dt_cap < 1e10;
+    int erat_n = -1;          // n_e of the last e_ratio value lane 0 computed
+    double erat_v = 0.0;
 
     while (t <= R.duration) {
         if (i >= max_steps) {
@@ -126,26 +146,39 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         lsteps += (uint64_t)n_live;
         const double T = (R.T_start_C + R.T_rate * t) + 273.15;
         const double kT = R.k_b * T;
-        const double A = R.s * exp_own(-(R.E_cb / kT));
-        const double B1 = b1_is_a ? A : R.b * exp_own(-(R.E_loc_1 / kT));
-        const double B2 = b2_is_b1 ? B1 : R.b * exp_own(-(R.E_loc_2 / kT));
+        const double A = boltzmann(R.s, s_ok, R.E_cb, kT);
+        const double B1 = b1_is_a ? A : boltzmann(R.b, b_ok, R.E_loc_1, kT);
+        const double B2 = b2_is_b1 ? B1 : boltzmann(R.b, b_ok, R.E_loc_2, kT);
         const uint32_t blk = (uint32_t)i >> 2;
         const int word = i & 3;
         auto rate_of = [&](int p) -> double {
             if (p >= n_live) return 0.0;
             const uint32_t st = est[p];
+            const float w = ew[p];            // read beside est (one LDS round trip)
             if ((st >> 16) == kNoHole) return 0.0;
             bool ch = always2;
             if (two_ch)
                 ch = (uint64_t)pick_word(philox10(blk, st & 0xFFFFu, prun, c3c, k0, k1), word) <
                      ret_thr;
-            return __builtin_fma(ch ? B2 : B1, (double)ew[p], A);
+            return __builtin_fma(ch ? B2 : B1, (double)w, A);
         };
         const int rows = (n_live + kWave - 1) / kWave;
         double K = 0.0;
-        for (int r = 0; r < rows; ++r) {
-            const double v = row_scan_d(rate_of(r * kWave + lane), lane);
-            const double rt = __shfl(v, kWave - 1, kWave);
+        // row totals (the scan's lane-63 values), two rows at a time (independent
+        // chains), added in order
+        int r = 0;
+        for (; r + 1 < rows; r += 2) {
+            const double ra = row_total_d(rate_of(r * kWave + lane));
+            const double rb = row_total_d(rate_of((r + 1) * kWave + lane));
+            if (lane == 0) {
+                rtot[r] = ra;
+                rtot[r + 1] = rb;
+            }
+            K = K + ra;
+            K = K + rb;
+        }
+        if (r < rows) {
+            const double rt = row_total_d(rate_of(r * kWave + lane));
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
         }
@@ -154,7 +187,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         const double lam = (R.D == 0.0 || n_live + filled == N_int)
                                ? 1e-20
                                : (R.D / R.D0) * (double)(N_int - (n_live + filled));
-        const double dt_fill = (1.0 / lam) * exp1_of(sb.z);
+        const double dt_fill = fill_never ? __builtin_inf() : (1.0 / lam) * exp1_of(sb.z);
         double dt_rec = dt_fill;
         if (n_live > 0) dt_rec = K > 0.0 ? exp1_of(sb.x) / K : __builtin_inf();
         const double dt = fmin(fmin(dt_fill, dt_rec), dt_cap);
@@ -226,7 +259,13 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         if (lane == 0) {
             lum[i] = ev;
             if (tax) tax[i] = t;
-            if (erat) erat[i] = (double)(n_live + filled) / R.N_e;
+            if (erat) {
+                if (n_live + filled != erat_n) {
+                    erat_n = n_live + filled;
+                    erat_v = (double)erat_n / R.N_e;
+                }
+                erat[i] = erat_v;
+            }
         }
         ++i;
         if (R.duration != 0.0 && t >= R.duration) break;

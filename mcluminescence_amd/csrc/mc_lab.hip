@@ -156,12 +156,15 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
             ch_step = (uint64_t)pick_word(philox10(blk, kSharedChanCtr, prun, c3c, k0, k1), word) <
                       ret_thr;
         auto rate_of = [&](int p) -> double {
-            if (p >= n_e || enn[p] == kNoHole) return 0.0;
+            if (p >= n_e) return 0.0;
+            const uint32_t nn = enn[p];
+            const float w = elec[p].w;        // read beside enn (one LDS round trip)
+            if (nn == kNoHole) return 0.0;
             bool ch = ch_step;
             if (two_ch && !shared_ch)
                 ch = (uint64_t)pick_word(philox10(blk, (uint32_t)p, prun, c3c, k0, k1), word) <
                      ret_thr;
-            return __builtin_fma(ch ? B2 : B1, (double)elec[p].w, A);
+            return __builtin_fma(ch ? B2 : B1, (double)w, A);
         };
         // one electron-hole pair (Box.add_electron); false when the cluster is full
         auto add_pair = [&](bool exact) -> bool {
@@ -236,9 +239,21 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
         };
         const int rows = (n_e + kWave - 1) / kWave;
         double K = 0.0;
-        for (int r = 0; r < rows; ++r) {
-            const double v = row_scan_d(rate_of(r * kWave + lane), lane);
-            const double rt = __shfl(v, kWave - 1, kWave);
+        // row totals (the scan's lane-63 values), two rows at a time (independent
+        // chains), added in order
+        int r = 0;
+        for (; r + 1 < rows; r += 2) {
+            const double ra = row_total_d(rate_of(r * kWave + lane));
+            const double rb = row_total_d(rate_of((r + 1) * kWave + lane));
+            if (lane == 0) {
+                rtot[r] = ra;
+                rtot[r + 1] = rb;
+            }
+            K = K + ra;
+            K = K + rb;
+        }
+        if (r < rows) {
+            const double rt = row_total_d(rate_of(r * kWave + lane));
             if (lane == 0) rtot[r] = rt;
             K = K + rt;
         }
