@@ -135,8 +135,28 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
     // -ln(1 - 2^-33)): with dt_cap below that it is never the minimum, so the
     // step does not draw it (dt_fill = +inf gives the same dt and the same event).
     const bool fill_never = R.D == 0.0 && dt_cap < 1e10;
-    int erat_n = -1;          // n_e of the last e_ratio value lane 0 computed
+    int erat_n = -1;          // n_e of the last e_ratio value computed
     double erat_v = 0.0;
+    // The step draws depend on the step index only: lane j holds those of step
+    // sb_base + j (U_w, E_t and E_f, each computed as the step would), 64 steps
+    // in one parallel evaluation; a step reads its own with scalar reads.
+    int sb_base = -kWave;
+    uint32_t l_uw = 0u;
+    double l_et = 0.0, l_ef = 0.0;
+    // The rows go out 64 at a time: lane j keeps row out_base + j until the
+    // block is full or the loop ends, then every lane stores its own (coalesced).
+    int out_base = 0;
+    uint32_t o_lum = 0u;
+    double o_t = 0.0, o_er = 0.0;
+    auto flush_rows = [&](int upto) {
+        const int row = out_base + lane;
+        if (row < upto) {
+            lum[row] = (uint8_t)o_lum;
+            if (tax) tax[row] = o_t;
+            if (erat) erat[row] = o_er;
+        }
+        out_base = upto;
+    };
 
     while (t <= R.duration) {
         if (i >= max_steps) {
@@ -183,13 +203,22 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             K = K + rt;
         }
         event_fence<GM>();
-        const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
+        if (i - sb_base >= kWave) {
+            const u32x4 o = philox10((uint32_t)(i + lane), 0xFFFFFFFFu, prun, c3f, k0, k1);
+            l_uw = o.y;
+            l_et = exp1_of(o.x);
+            l_ef = fill_never ? 0.0 : exp1_of(o.z);
+            sb_base = i;
+        }
+        const int js = i - sb_base;
+        const uint32_t u_w = lane_value(l_uw, js);
         const double lam = (R.D == 0.0 || n_live + filled == N_int)
                                ? 1e-20
                                : (R.D / R.D0) * (double)(N_int - (n_live + filled));
-        const double dt_fill = fill_never ? __builtin_inf() : (1.0 / lam) * exp1_of(sb.z);
+        const double dt_fill =
+            fill_never ? __builtin_inf() : (1.0 / lam) * lane_value_d(l_ef, js);
         double dt_rec = dt_fill;
-        if (n_live > 0) dt_rec = K > 0.0 ? exp1_of(sb.x) / K : __builtin_inf();
+        if (n_live > 0) dt_rec = K > 0.0 ? lane_value_d(l_et, js) / K : __builtin_inf();
         const double dt = fmin(fmin(dt_fill, dt_rec), dt_cap);
         t += dt;
         uint8_t ev = 0;
@@ -201,7 +230,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             }
         } else if (dt == dt_rec) {
             // the recombining electron: first running sum above U_w K
-            const double target = (double)sb.y * 0x1p-32 * K;
+            const double target = (double)u_w * 0x1p-32 * K;
             int pw = -1;
             double carry = 0.0;
             for (int r = 0; r < rows; ++r) {
@@ -256,21 +285,21 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             }
             ev = 1;
         }
-        if (lane == 0) {
-            lum[i] = ev;
-            if (tax) tax[i] = t;
-            if (erat) {
-                if (n_live + filled != erat_n) {
-                    erat_n = n_live + filled;
-                    erat_v = (double)erat_n / R.N_e;
-                }
-                erat[i] = erat_v;
-            }
+        if (erat && n_live + filled != erat_n) {
+            erat_n = n_live + filled;
+            erat_v = (double)erat_n / R.N_e;
+        }
+        if (lane == i - out_base) {
+            o_lum = ev;
+            o_t = t;
+            o_er = erat_v;
         }
         ++i;
+        if (i - out_base == kWave) flush_rows(i);
         if (R.duration != 0.0 && t >= R.duration) break;
     }
     const int used = i;
+    flush_rows(used);
     for (int r = used + lane; r < max_steps; r += kWave) {
         lum[r] = 0;
         if (tax) tax[r] = 0.0;
