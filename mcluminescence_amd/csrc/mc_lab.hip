@@ -138,6 +138,13 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
     int i = 0, k_obs = 0, fills = 0;
     int32_t status = 0;
     uint64_t lsteps = 0, recomb = 0;
+    // The step draws depend on the step index only: lane j holds those of step
+    // sb_base + j (E_t, U_w, E_f and the retrap word, each computed as the step
+    // would), 64 steps in one parallel evaluation; a step reads its own with
+    // scalar reads.
+    int sb_base = -kWave;
+    uint32_t l_uw = 0u, l_rw = 0u;
+    double l_et = 0.0, l_ef = 0.0;
     while (k_obs < n_obs) {
         if ((int64_t)i >= a.max_steps) {
             status = -1;
@@ -258,12 +265,20 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
             K = K + rt;
         }
         __syncthreads();
-        const u32x4 sb = philox10((uint32_t)i, 0xFFFFFFFFu, prun, c3f, k0, k1);
+        if (i - sb_base >= kWave) {
+            const u32x4 o = philox10((uint32_t)(i + lane), 0xFFFFFFFFu, prun, c3f, k0, k1);
+            l_et = lab_exp1(o.x);
+            l_uw = o.y;
+            l_ef = lab_exp1(o.z);
+            l_rw = o.w;
+            sb_base = i;
+        }
+        const int js = i - sb_base;
         const double lam = (R.D == 0.0 || n_e == N_int) ? 1e-20
                                                          : (R.D / R.D0) * (double)(N_int - n_e);
-        const double dt_fill = (1.0 / lam) * lab_exp1(sb.z);
+        const double dt_fill = (1.0 / lam) * lane_value_d(l_ef, js);
         double dt_rec = dt_fill;
-        if (n_e > 0) dt_rec = K > 0.0 ? lab_exp1(sb.x) / K : __builtin_inf();
+        if (n_e > 0) dt_rec = K > 0.0 ? lane_value_d(l_et, js) / K : __builtin_inf();
         const double dt = fmin(dt_rec, dt_fill);
         t += dt;
         if (dt == dt_fill) {
@@ -282,7 +297,7 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                     if (m != 0) pw = r * kWave + (int)__builtin_ctzll(m);
                 }
             } else {
-                const double target = (double)sb.y * 0x1p-32 * K;
+                const double target = (double)lane_value(l_uw, js) * 0x1p-32 * K;
                 double carry = 0.0;
                 for (int r = 0; r < rows; ++r) {
                     const double c2 = carry + rtot[r];
@@ -335,7 +350,7 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                 __syncthreads();
             }
             // legacy retrapping: a new pair after the recombination
-            if (retrap && (uint64_t)sb.w < R.retrap_thr && !add_pair(true)) {
+            if (retrap && (uint64_t)lane_value(l_rw, js) < R.retrap_thr && !add_pair(true)) {
                 status = -2;
                 break;
             }
