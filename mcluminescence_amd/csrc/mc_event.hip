@@ -171,16 +171,18 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         const double B2 = b2_is_b1 ? B1 : boltzmann(R.b, b_ok, R.E_loc_2, kT);
         const uint32_t blk = (uint32_t)i >> 2;
         const int word = i & 3;
+        // k_e of slot p, 0 for slots past the live ones and electrons with no hole;
+        // branch-free: every slot of a row lies within e_cap, so the reads are in
+        // bounds, and the selects drop what the slots past n_live hold
         auto rate_of = [&](int p) -> double {
-            if (p >= n_live) return 0.0;
             const uint32_t st = est[p];
-            const float w = ew[p];            // read beside est (one LDS round trip)
-            if ((st >> 16) == kNoHole) return 0.0;
+            const float w = ew[p];
             bool ch = always2;
             if (two_ch)
                 ch = (uint64_t)pick_word(philox10(blk, st & 0xFFFFu, prun, c3c, k0, k1), word) <
                      ret_thr;
-            return __builtin_fma(ch ? B2 : B1, (double)w, A);
+            const double k = __builtin_fma(ch ? B2 : B1, (double)w, A);
+            return p < n_live && (st >> 16) != kNoHole ? k : 0.0;
         };
         const int rows = (n_live + kWave - 1) / kWave;
         double K = 0.0;

@@ -162,16 +162,18 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
         if (two_ch && shared_ch)
             ch_step = (uint64_t)pick_word(philox10(blk, kSharedChanCtr, prun, c3c, k0, k1), word) <
                       ret_thr;
+        // k_e of slot p, 0 for slots past the live ones and electrons with no hole;
+        // branch-free: every slot of a row lies within e_cap, so the reads are in
+        // bounds, and the selects drop what the slots past n_e hold
         auto rate_of = [&](int p) -> double {
-            if (p >= n_e) return 0.0;
             const uint32_t nn = enn[p];
-            const float w = elec[p].w;        // read beside enn (one LDS round trip)
-            if (nn == kNoHole) return 0.0;
+            const float w = elec[p].w;
             bool ch = ch_step;
             if (two_ch && !shared_ch)
                 ch = (uint64_t)pick_word(philox10(blk, (uint32_t)p, prun, c3c, k0, k1), word) <
                      ret_thr;
-            return __builtin_fma(ch ? B2 : B1, (double)w, A);
+            const double k = __builtin_fma(ch ? B2 : B1, (double)w, A);
+            return p < n_e && nn != kNoHole ? k : 0.0;
         };
         // one electron-hole pair (Box.add_electron); false when the cluster is full
         auto add_pair = [&](bool exact) -> bool {
