@@ -141,6 +141,24 @@ def cpu_event_lab(seconds: float) -> dict:
     event = {"value": steps / el, "unit": "event steps/s", "cores": 1, "kind": "port",
              "sample": f"{done} clusters of the event workload in {el:.1f} s, numpy "
                        "restatement of simulate.py:36-92"}
+    # the 16 x 500 TL glow (300 -> 700 K at 1 K/s) through the same loop, step cap 0.1 s
+    from mcluminescence_amd import workloads as W
+    grun = W.tl_glow_16x500(sims=1)[0]
+    gexp, gphys = E.run_params_to_cfg(grun)
+    gexp["duration"] = grun.n_steps * grun.dt
+    gexp["max_dt"] = grun.dt * grun.T_rate
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        E.simulate_run(gexp, gphys, sims=1, steps=8192, seed=0x61077, first_sim=done)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or done >= 16:
+            break
+    tl_glow = {"value": el / done * 16 * 1e3, "unit": "ms per run", "cores": 1, "kind": "port",
+               "sample": f"{done} of the 16 clusters of the 16 x 500 TL glow in {el:.1f} s "
+                         "(the run's time scaled to 16 clusters), numpy restatement of "
+                         "simulate.py:36-92 with the step cap 0.1 s"}
     lexp, lphys, rows = lab_workload()
     rng = np.random.default_rng(11)
     done = 0
@@ -154,7 +172,7 @@ def cpu_event_lab(seconds: float) -> dict:
     lab = {"value": done * len(rows) / el, "unit": "lab experiments/s", "cores": 1,
            "kind": "port", "sample": f"{done} passes over the {len(rows)} TL rows in {el:.1f} s, "
                                      "numpy restatement of TL_lab (tl_trap_lab.py:65-107)"}
-    return {"event": event, "lab": lab}
+    return {"event": event, "lab": lab, "tl_glow_16x500": tl_glow}
 
 
 EVENT_DURATION = 160.0      # s: 0 -> 800 degC at 5 K/s
@@ -365,6 +383,14 @@ def cpu_harness(seconds: float) -> dict:
             sims = min(4096, sims * 4)
         out[label] = {"value": live / el, "unit": "electron-steps/s", "cores": th,
                       "sample": f"{sims} of the 4096 decay clusters in {el:.2f} s"}
+    # the 16 x 500 TL glow (all 16 clusters, 4000 steps), whole run
+    for label, th in (("tl_glow_16x500_1core", 1), ("tl_glow_16x500_allcores", cores)):
+        t0 = time.perf_counter()
+        _, lv = orc.run_fixed_dt(W.tl_glow_16x500(), threads=th)
+        el = time.perf_counter() - t0
+        out[label] = {"value": el * 1e3, "unit": "ms per run", "cores": th,
+                      "electron_steps_per_s": int(lv.sum()) / el,
+                      "sample": "the whole 16 x 500 TL glow (16 clusters, 4000 steps)"}
     runs = W.uneven_sweep()
     k = 64
     while True:
