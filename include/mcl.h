@@ -142,6 +142,12 @@ int         mcl_cluster_limits(const void* ctx, int32_t* max_electrons, int32_t*
  *                     0: the kernels that compute it from w every step (results
  *                     identical; ramped and two-channel launches always take
  *                     those).
+ *   "quiet_window"    1 (default): fixed-dt launches (or size classes) with a
+ *                     ramped run take the kernels with the quiet window: after
+ *                     two steps in which no electron fired, the steps left in
+ *                     the Philox block (up to three) are tested at once and
+ *                     skipped together when no electron fires in them; 0: the
+ *                     kernels without it (results identical).
  * No counterpart in the reference (it has no kernels); the autotuner
  * (mcluminescence_amd/autotune.py) sets them. */
 int         mcl_tune_set(void* ctx, const char* key, double value);

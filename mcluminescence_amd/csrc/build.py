@@ -18,7 +18,8 @@ PKG = HERE.parent
 ROOT = PKG.parent
 OUT = PKG / "libmcl.so"
 
-SOURCES = ["mc_fixed_dt.hip", "mc_fixed_dt_global.hip", "mc_event.hip", "mc_lab.hip", "glow.hip",
+SOURCES = ["mc_fixed_dt.hip", "mc_fixed_dt_window.hip", "mc_fixed_dt_global.hip",
+           "mc_fixed_dt_global_window.hip", "mc_event.hip", "mc_lab.hip", "glow.hip",
            "curves.hip", "mcl_abi.hip", "mlp.hip", "mlp_fused.hip", "mlp_abi.hip", "de.hip"]
 
 # -ffp-contract=off: every fused multiply-add in a decision path is an explicit

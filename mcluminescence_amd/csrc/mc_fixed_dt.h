@@ -127,8 +127,10 @@ __device__ __forceinline__ void queue_sync()
 // and results of the exchanges).  FW: the build with the fixed-threshold path
 // (isothermal one-channel runs keep fire thresholds in their w slots); without
 // it every run computes its thresholds from w, and the code of that path is not
-// in the kernel.  Both builds give the same counts for every run.
-template <int NW, bool GM, bool FW>
+// in the kernel.  QW: the build with the quiet window (after quiet steps, the rest
+// of a Philox block tested at once).  Every build gives the same counts for every
+// run.
+template <int NW, bool GM, bool FW, bool QW>
 __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
                                             unsigned char* lds_raw)
 {
@@ -364,6 +366,136 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
     float l_xA = 0.0f, l_b1dt = 0.0f, l_b2dt = 0.0f;   // step scalars of step sc_base + lane
     uint32_t l_thr = 0u, l_always = 0u;
     int sc_base = -kWave;
+    // The step scalars of step m >= 1 of a ramp, from lane m - sc_base (the 64 steps
+    // from sc_base are evaluated in parallel, refilled from m when m leaves them).
+    auto ramp_scalars = [&](int m, int last) -> StepScalars {
+        if (m < sc_base || last - sc_base >= kWave) {
+            const StepScalars sc = scalars_at(m + lane);
+            l_xA = sc.xA;
+            l_b1dt = sc.b1dt;
+            l_b2dt = sc.b2dt;
+            l_thr = sc.tA.thr;
+            l_always = sc.tA.always ? 1u : 0u;
+            sc_base = m;
+        }
+        const int j = m - sc_base;
+        StepScalars sc;
+        sc.xA = lane_value(l_xA, j);
+        sc.b1dt = lane_value(l_b1dt, j);
+        sc.b2dt = lane_value(l_b2dt, j);
+        sc.tA.thr = lane_value(l_thr, j);
+        sc.tA.always = lane_value(l_always, j) != 0u;
+        return sc;
+    };
+    // Quiet window: the steps n0 .. n0 + W - 1 (n0 >= 1, within one Philox block, so
+    // one block per electron holds all their draws) tested at once, each live
+    // electron's draw of each step against that step's threshold exactly as pass A
+    // tests it (a stale electron against its old w).  Returns how many steps from
+    // n0 no electron of the cluster fires in (W: none); those steps change nothing
+    // but the tallies, and the next step that runs finds its cached rows' blocks in
+    // the register cache.
+    auto quiet_window = [&](int n0, int W) -> int {
+        const uint32_t wblk = (uint32_t)n0 >> 2;
+        const int wd = n0 & 3;
+        const int wrows = (n_slots + kWave - 1) / kWave;
+        const int per = (wrows + NW - 1) / NW;      // rows of a wave, at most (all waves)
+        // only rows whose blocks the register cache keeps (the next step reads them
+        // there); with more rows a wave, the steps run one by one (the same choice
+        // in every wave: a window holds an exchange)
+        if (per > 2 * kCachePairs) return 0;
+        int w_lo = 0, wnr = wrows, w_end = n_slots;
+        if constexpr (NW > 1) {
+            w_lo = min(wrows, wv * per);
+            wnr = min(wrows, w_lo + per) - w_lo;
+            w_end = min(n_slots, (w_lo + wnr) * kWave);
+        }
+        if (R.T_rate != 0.0) (void)ramp_scalars(n0, n0 + W - 1);   // the window's steps in the lanes
+        // step scalars of step n0 + k
+        auto scal = [&](int k) -> StepScalars {
+            if (R.T_rate == 0.0) return StepScalars{xA, b1dt, b2dt, tA};
+            return ramp_scalars(n0 + k, n0 + k);
+        };
+        if ((int)wblk != cache_blk) {
+            cache_blk = (int)wblk;
+            cache_pairs = 0u;
+        }
+        // the first step in which an electron of this wave fires (W: none); a pair
+        // tests only the steps before the first one an earlier pair found
+        int lim = W;
+        auto window_pair = [&](int r, auto slot_tag) {
+            constexpr int q = decltype(slot_tag)::value;
+            const int p0 = (w_lo + r) * kWave + lane, p1 = p0 + kWave;
+            const uint32_t st0 = p0 < w_end ? est[p0] : kDeadSlot;
+            const uint32_t st1 = p1 < w_end ? est[p1] : kDeadSlot;
+            const float v0 = p0 < w_end ? ew[p0] : 0.0f, v1 = p1 < w_end ? ew[p1] : 0.0f;
+            const uint32_t o0 = st0 & 0xFFFFu, o1 = st1 & 0xFFFFu;
+            u32x4 fa, fb;
+            uint32_t cb = 0u;
+            bool hit = false;
+            if constexpr (q >= 0) hit = ((cache_pairs >> q) & 1u) != 0u;
+            if (hit) {
+                if constexpr (q >= 0) {
+                    fa = cfa[q];
+                    fb = cfb[q];
+                    cb = cch[q];
+                }
+            } else {
+                fa = philox10(wblk, o0, prun, c3f, k0, k1);
+                fb = philox10(wblk, o1, prun, c3f, k0, k1);
+                if (two_ch) {
+                    const u32x4 ca = philox10(wblk, o0, prun, c3c, k0, k1);
+                    const u32x4 cc = philox10(wblk, o1, prun, c3c, k0, k1);
+                    cb = (uint32_t)((uint64_t)ca.x < ret_thr) | (uint32_t)((uint64_t)ca.y < ret_thr) << 1 |
+                         (uint32_t)((uint64_t)ca.z < ret_thr) << 2 | (uint32_t)((uint64_t)ca.w < ret_thr) << 3 |
+                         (uint32_t)((uint64_t)cc.x < ret_thr) << 4 | (uint32_t)((uint64_t)cc.y < ret_thr) << 5 |
+                         (uint32_t)((uint64_t)cc.z < ret_thr) << 6 | (uint32_t)((uint64_t)cc.w < ret_thr) << 7;
+                }
+                if constexpr (q >= 0) {
+                    cfa[q] = fa;
+                    cfb[q] = fb;
+                    cch[q] = cb;
+                    cache_pairs |= 1u << q;
+                }
+            }
+            for (int k = 0; k < lim; ++k) {
+                const StepScalars sc = scal(k);
+                const int d = wd + k;
+                const bool ch0 = two_ch ? ((cb >> d) & 1u) != 0u : always2;
+                const bool ch1 = two_ch ? ((cb >> (4 + d)) & 1u) != 0u : always2;
+                FireThr t0, t1;
+                if (fixedw) {
+                    t0 = slot_threshold(v0);
+                    t1 = slot_threshold(v1);
+                } else {
+                    const float x0 = __builtin_fmaf(ch0 ? sc.b2dt : sc.b1dt, v0, sc.xA);
+                    const float x1 = __builtin_fmaf(ch1 ? sc.b2dt : sc.b1dt, v1, sc.xA);
+                    t0 = x0 != sc.xA ? fire_threshold(x0) : sc.tA;
+                    t1 = x1 != sc.xA ? fire_threshold(x1) : sc.tA;
+                }
+                const bool f0 = (st0 >> 16) != kNoHole && (t0.always || pick_word(fa, d) < t0.thr);
+                const bool f1 = (st1 >> 16) != kNoHole && (t1.always || pick_word(fb, d) < t1.thr);
+                if (ballot(f0 || f1) != 0) {
+                    lim = k;
+                    break;
+                }
+            }
+        };
+        if (NW == 1 || wnr > 0) window_pair(0, CacheSlot<0>{});
+        if (wnr > 2) window_pair(2, CacheSlot<1>{});
+        if (wnr > 4) window_pair(4, CacheSlot<2>{});
+        if (wnr > 6) window_pair(6, CacheSlot<3>{});
+        if constexpr (NW > 1) {
+            const uint32_t* x = exchange((uint32_t)lim);
+            uint32_t m = (uint32_t)W;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) m = x[w] < m ? x[w] : m;
+            lim = (int)uniform(m);
+        }
+        return lim;
+    };
+    int quiet_run = 0;           // steps since the last one with a candidate in some wave
+    int quiet_need = 2;          // quiet steps before a window; doubled after a window
+                                 // whose first step fires, back to 2 after a whole one
     int n = 0;
     while (n < step_limit && n_live > 0) {
         if (n_live <= a.lookahead_max) {
@@ -376,27 +508,33 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
             n += f;
             steps_run = n;
             if (!found) continue;
+        } else if (QW && quiet_run >= quiet_need && (n & 3) <= 2) {
+            // a quiet stretch: the rest of the block at once
+            const int W = min(4 - (n & 3), step_limit - n);
+            const int f = quiet_window(n, W);
+            for (int k = 0; k < f; ++k) {
+                lsteps += (uint64_t)n_live;
+                finish_step(0u);
+            }
+            n += f;
+            steps_run = n;
+            quiet_run += f;
+            if (f == 0) quiet_need = min(2 * quiet_need, 64);
+            if (f == W) {
+                quiet_need = 2;
+                continue;
+            }
         }
         lsteps += (uint64_t)n_live;
         steps_run = n + 1;
         if (n > 0 && R.T_rate != 0.0) {
             // a ramp: lane j holds the scalars of step sc_base + j (64 steps in one
             // parallel evaluation instead of 64 serial ones)
-            if (n < sc_base || n - sc_base >= kWave) {
-                const StepScalars sc = scalars_at(n + lane);
-                l_xA = sc.xA;
-                l_b1dt = sc.b1dt;
-                l_b2dt = sc.b2dt;
-                l_thr = sc.tA.thr;
-                l_always = sc.tA.always ? 1u : 0u;
-                sc_base = n;
-            }
-            const int j = n - sc_base;
-            xA = lane_value(l_xA, j);
-            b1dt = lane_value(l_b1dt, j);
-            b2dt = lane_value(l_b2dt, j);
-            tA.thr = lane_value(l_thr, j);
-            tA.always = lane_value(l_always, j) != 0u;
+            const StepScalars sc = ramp_scalars(n, n);
+            xA = sc.xA;
+            b1dt = sc.b1dt;
+            b2dt = sc.b2dt;
+            tA = sc.tA;
         }
         const uint32_t blk = (uint32_t)n >> 2;
         const int word = n & 3;
@@ -511,7 +649,9 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
         uint64_t rowmask = rows_any(cand, nr);
         uint32_t wins = 0;          // this wave's winners in this step
         uint32_t step_wins = 0;     // the cluster's
-        if (any_wave(rowmask != 0)) {
+        const bool any_cand = any_wave(rowmask != 0);
+        quiet_run = any_cand ? 0 : quiet_run + 1;
+        if (any_cand) {
             for (int round = 0;; ++round) {
                 if (round > 0) {
                     // claims of this round's candidates (losers accepted for a retry)
@@ -787,13 +927,13 @@ __device__ __forceinline__ void run_cluster(const LaunchArgs& a, const int c,
 // from order[] (null: 0 .. n_order-1) through a global atomic ticket until the
 // list is exhausted.  Every workgroup leaves the loop when the ticket passes
 // n_order.  The global-memory form always runs persistent.
-template <bool GM, bool FW>
+template <bool GM, bool FW, bool QW>
 __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (a.queue == nullptr) {
         const int i = blockIdx.x;
-        if (i < a.n_order) run_cluster<1, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        if (i < a.n_order) run_cluster<1, GM, FW, QW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         return;
     }
     for (;;) {
@@ -801,7 +941,7 @@ __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
         if (threadIdx.x == 0) i = (int)atomicAdd(a.queue, 1u);
         i = __shfl(i, 0);
         if (i >= a.n_order) break;
-        run_cluster<1, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        run_cluster<1, GM, FW, QW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         __syncthreads();
     }
 }
@@ -810,13 +950,13 @@ __global__ __launch_bounds__(64, GM ? 2 : 1) void fixed_dt_kernel(LaunchArgs a)
 // most 256 VGPRs a wave, as the one-wave kernel).  The persistent ticket goes
 // through LDS (the word after the exchange slots), read by every wave between
 // two barriers, so all waves take the same cluster and leave the loop together.
-template <int NW, bool GM, bool FW>
+template <int NW, bool GM, bool FW, bool QW>
 __global__ __launch_bounds__(NW * 64, 2) void fixed_dt_mw_kernel(LaunchArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (a.queue == nullptr) {
         const int i = blockIdx.x;
-        if (i < a.n_order) run_cluster<NW, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        if (i < a.n_order) run_cluster<NW, GM, FW, QW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         return;
     }
     uint32_t* const ticket =
@@ -828,63 +968,85 @@ __global__ __launch_bounds__(NW * 64, 2) void fixed_dt_mw_kernel(LaunchArgs a)
         const int i = (int)uniform(*ticket);
         __syncthreads();
         if (i >= a.n_order) break;
-        run_cluster<NW, GM, FW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
+        run_cluster<NW, GM, FW, QW>(a, a.order != nullptr ? a.order[i] : i, lds_raw);
         __syncthreads();
     }
 }
 
-template <bool GM, bool FW>
+template <bool GM, bool FW, bool QW>
 const void* fixed_dt_fn_t(int waves)
 {
-    return waves == 8 ? (const void*)fixed_dt_mw_kernel<8, GM, FW>
-         : waves == 4 ? (const void*)fixed_dt_mw_kernel<4, GM, FW>
-         : waves == 2 ? (const void*)fixed_dt_mw_kernel<2, GM, FW>
-                      : (const void*)fixed_dt_kernel<GM, FW>;
+    return waves == 8 ? (const void*)fixed_dt_mw_kernel<8, GM, FW, QW>
+         : waves == 4 ? (const void*)fixed_dt_mw_kernel<4, GM, FW, QW>
+         : waves == 2 ? (const void*)fixed_dt_mw_kernel<2, GM, FW, QW>
+                      : (const void*)fixed_dt_kernel<GM, FW, QW>;
 }
 
-template <bool GM>
+// The build of the NW-wave kernel with the quiet window or not (QW) and with the
+// fixed-threshold path or not (fw).
+template <bool GM, bool QW>
 const void* fixed_dt_fn(int waves, bool fw)
 {
-    return fw ? fixed_dt_fn_t<GM, true>(waves) : fixed_dt_fn_t<GM, false>(waves);
+    return fw ? fixed_dt_fn_t<GM, true, QW>(waves) : fixed_dt_fn_t<GM, false, QW>(waves);
 }
 
-template <bool GM, bool FW>
+template <bool GM, bool FW, bool QW>
 void launch_fixed_dt_w(const LaunchArgs& a, dim3 g, unsigned lds, int waves, hipStream_t stream)
 {
     if (waves == 8)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<8, GM, FW>), g, dim3(8 * kWave), lds, stream, a);
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<8, GM, FW, QW>), g, dim3(8 * kWave), lds, stream, a);
     else if (waves == 4)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<4, GM, FW>), g, dim3(4 * kWave), lds, stream, a);
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<4, GM, FW, QW>), g, dim3(4 * kWave), lds, stream, a);
     else if (waves == 2)
-        hipLaunchKernelGGL((fixed_dt_mw_kernel<2, GM, FW>), g, dim3(2 * kWave), lds, stream, a);
+        hipLaunchKernelGGL((fixed_dt_mw_kernel<2, GM, FW, QW>), g, dim3(2 * kWave), lds, stream, a);
     else
-        hipLaunchKernelGGL((fixed_dt_kernel<GM, FW>), g, dim3(kWave), lds, stream, a);
+        hipLaunchKernelGGL((fixed_dt_kernel<GM, FW, QW>), g, dim3(kWave), lds, stream, a);
 }
 
-template <bool GM>
+// The builds with the quiet window (QW) or without are instantiated in separate
+// sources (mc_fixed_dt*.hip without it, mc_fixed_dt*_window.hip with it), which
+// the launch functions choose between by a.quiet_window.
+template <bool GM, bool QW>
 hipError_t launch_fixed_dt_t(const LaunchArgs& a, int grid, unsigned lds, int waves,
                              hipStream_t stream)
 {
     const dim3 g((unsigned)grid);
     if (a.fixed_thr != 0)
-        launch_fixed_dt_w<GM, true>(a, g, lds, waves, stream);
+        launch_fixed_dt_w<GM, true, QW>(a, g, lds, waves, stream);
     else
-        launch_fixed_dt_w<GM, false>(a, g, lds, waves, stream);
+        launch_fixed_dt_w<GM, false, QW>(a, g, lds, waves, stream);
     return hipGetLastError();
 }
 
 // Resident workgroups per CU of the NW-wave kernel with this much dynamic LDS:
-// the smaller count of its two builds (the launch may take either).
-template <bool GM>
+// the smaller count of its two fixed-threshold builds.
+template <bool GM, bool QW>
 hipError_t fixed_dt_occupancy_t(size_t lds, int waves, int* blocks_per_cu)
 {
     int b0 = 0, b1 = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, fixed_dt_fn<GM>(waves, false),
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, fixed_dt_fn<GM, QW>(waves, false),
                                                                 waves * kWave, lds);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, fixed_dt_fn<GM>(waves, true),
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, fixed_dt_fn<GM, QW>(waves, true),
                                                          waves * kWave, lds);
     *blocks_per_cu = b0 < b1 ? b0 : b1;
+    return e;
+}
+
+// Dynamic-LDS limits of the builds of one form (GM) and window choice (QW).
+template <bool GM, bool QW>
+hipError_t prepare_fixed_dt_t(int bytes_one_wave, int max_lds)
+{
+    hipError_t e = hipSuccess;
+    for (int w : {1, 2, 4, 8}) {
+        for (bool fw : {false, true}) {
+            int b = bytes_one_wave + fixed_dt_waves_lds(w);
+            b = b < max_lds ? b : max_lds;
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(fixed_dt_fn<GM, QW>(w, fw),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, b);
+        }
+    }
     return e;
 }
 

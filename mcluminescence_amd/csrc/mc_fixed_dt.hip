@@ -1,5 +1,6 @@
 // mc_fixed_dt.hip — the fixed-dt kernels whose clusters live wholly in LDS (the
-// cluster body, its layout and the kernel templates are in mc_fixed_dt.h).
+// cluster body, its layout and the kernel templates are in mc_fixed_dt.h), the
+// builds without the quiet window; mc_fixed_dt_window.hip holds those with it.
 #include "mc_fixed_dt.h"
 
 #include <algorithm>
@@ -16,23 +17,17 @@ __global__ void reset_tickets_kernel(uint32_t* tickets, int n)
 
 hipError_t prepare_fixed_dt(int lds_bytes, int max_lds)
 {
-    hipError_t e = hipSuccess;
-    for (int w : {1, 2, 4, 8}) {
-        for (bool fw : {false, true}) {
-            if (e == hipSuccess)
-                e = hipFuncSetAttribute(fixed_dt_fn<false>(w, fw),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        std::min(lds_bytes + fixed_dt_waves_lds(w), max_lds));
-        }
-    }
+    hipError_t e = prepare_fixed_dt_t<false, false>(lds_bytes, max_lds);
+    if (e == hipSuccess) e = prepare_fixed_dt_window(lds_bytes, max_lds);
     return e;
 }
 
 hipError_t launch_fixed_dt(const LaunchArgs& a, int grid, int lds_bytes, int waves,
                            hipStream_t stream)
 {
-    return launch_fixed_dt_t<false>(a, grid, (unsigned)(lds_bytes + fixed_dt_waves_lds(waves)),
-                                    waves, stream);
+    if (a.quiet_window != 0) return launch_fixed_dt_window(a, grid, lds_bytes, waves, stream);
+    return launch_fixed_dt_t<false, false>(a, grid, (unsigned)(lds_bytes + fixed_dt_waves_lds(waves)),
+                                           waves, stream);
 }
 
 hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream)
@@ -44,8 +39,12 @@ hipError_t launch_reset_tickets(uint32_t* tickets, int n, hipStream_t stream)
 
 hipError_t fixed_dt_occupancy(int lds_bytes, int waves, int* blocks_per_cu)
 {
-    return fixed_dt_occupancy_t<false>((size_t)(lds_bytes + fixed_dt_waves_lds(waves)), waves,
-                                       blocks_per_cu);
+    int b0 = 0, b1 = 0;
+    hipError_t e = fixed_dt_occupancy_t<false, false>((size_t)(lds_bytes + fixed_dt_waves_lds(waves)),
+                                                      waves, &b0);
+    if (e == hipSuccess) e = fixed_dt_window_occupancy(lds_bytes, waves, &b1);
+    *blocks_per_cu = b0 < b1 ? b0 : b1;
+    return e;
 }
 
 }  // namespace mcl

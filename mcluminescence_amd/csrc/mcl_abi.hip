@@ -143,6 +143,7 @@ struct Plan {
     int32_t max_e = 0, max_h = 0;
     int32_t waves = 1;            // wavefronts per cluster of the one-workgroup-per-cluster launch
     int32_t fixed_thr = 0;        // some run keeps its fire thresholds (keeps_thresholds)
+    int32_t window = 0;           // some run takes the quiet window (uses_window)
     uint32_t flags = 0;
     // clusters beyond one workgroup's LDS (global-memory form): the plain launch is
     // persistent, ggrid workgroups with a scratch slice of gstride bytes each
@@ -162,6 +163,7 @@ struct Plan {
         int32_t max_e = 0, max_h = 0;
         int32_t waves = 1;
         int32_t fixed_thr = 0;          // some cluster's run keeps its fire thresholds
+        int32_t window = 0;             // some cluster's run takes the quiet window
         bool global = false;            // global-memory form: scratch at goff, gstride a slice
         int32_t e_lds = 0;
         int64_t gstride = 0;
@@ -250,6 +252,15 @@ int validate_run(Ctx* c, const mcl_run_params& p, int r, int32_t n_bins, mcl_geo
 bool keeps_thresholds(const Ctx* c, const mcl::RunDesc& d)
 {
     return c->fixed_thresholds != 0.0 && d.T_rate == 0.0 && !d.two_channel;
+}
+
+// Runs on a temperature ramp: a launch with one of them takes the kernels with the
+// quiet window (long stretches in which no electron fires, before and after the
+// glow peak; an isothermal decay fires in nearly every step until its last few
+// electrons, which the look-ahead covers).
+bool uses_window(const Ctx* c, const mcl::RunDesc& d)
+{
+    return c->quiet_window != 0.0 && d.T_rate != 0.0;
 }
 
 mcl::RunDesc make_desc(const mcl_run_params& p, const mcl_geometry& g, int run_id,
@@ -429,6 +440,7 @@ int build_classes(Ctx* c, Plan* p, const mcl_run_params* runs, int32_t n_runs,
             cl.max_h = std::max(cl.max_h, d.n_h);
             max_cells = std::max(max_cells, d.grid * d.grid * d.grid);
             if (keeps_thresholds(c, d)) cl.fixed_thr = 1;
+            if (uses_window(c, d)) cl.window = 1;
         }
         cl.h_cap = ((cl.max_h + 3) / 4) * 4 + 4;
         cl.e_cap = std::max(64, ((cl.max_e + 63) / 64) * 64);
@@ -531,6 +543,8 @@ int make_plan(Ctx* c, const mcl_run_params* runs, int32_t n_runs, const int32_t*
     p->flags = flags;
     for (int r = 0; r < n_runs; ++r)
         if (runs[r].sims > 0 && keeps_thresholds(c, desc[(size_t)r])) p->fixed_thr = 1;
+    for (int r = 0; r < n_runs; ++r)
+        if (runs[r].sims > 0 && uses_window(c, desc[(size_t)r])) p->window = 1;
     p->global = L.global;
     p->e_lds = L.e_lds;
     p->gstride = L.gstride;
@@ -616,6 +630,7 @@ mcl::LaunchArgs launch_args(const Plan* p, uint32_t* counts, uint64_t* live)
     a.g_cap = p->g_cap;
     a.wave_search_max = (int32_t)p->ctx->wave_search_max;
     a.lookahead_max = (int32_t)p->ctx->lookahead_max;
+    a.quiet_window = p->window;
     a.dump.enabled = 0;
     a.fixed_thr = p->fixed_thr;
     return a;
@@ -637,6 +652,7 @@ hipError_t enqueue_persistent(Plan* p, uint32_t* counts, uint64_t* live, hipStre
         a.n_order = cl.count;
         a.queue = p->d_tickets + k;
         a.fixed_thr = cl.fixed_thr;
+        a.quiet_window = cl.window;
         a.h_cap = cl.h_cap;
         a.e_cap = cl.e_cap;
         a.g_cap = cl.g_cap;
@@ -843,6 +859,12 @@ int mcl_tune_set(void* ctx, const char* key, double value)
         c->fixed_thresholds = value;
         return MCL_OK;
     }
+    if (std::strcmp(key, "quiet_window") == 0) {
+        if (!(value == 0.0 || value == 1.0))
+            return fail(c, MCL_EINVAL, "quiet_window must be 0 or 1 (got %g)", value);
+        c->quiet_window = value;
+        return MCL_OK;
+    }
     return fail(c, MCL_EINVAL, "unknown tuning key '%s'", key);
 }
 
@@ -880,6 +902,10 @@ int mcl_tune_get(const void* ctx, const char* key, double* value)
     }
     if (std::strcmp(key, "fixed_thresholds") == 0) {
         *value = c->fixed_thresholds;
+        return MCL_OK;
+    }
+    if (std::strcmp(key, "quiet_window") == 0) {
+        *value = c->quiet_window;
         return MCL_OK;
     }
     return MCL_EINVAL;
@@ -1450,7 +1476,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     // descriptors and cluster map through a fixed host staging block, one chunk
     // at a time (each copied and waited for before the block is refilled)
     alignas(64) unsigned char stage[16384];
-    int32_t fixed_thr = 0;
+    int32_t fixed_thr = 0, window = 0;
     auto upload = [&](size_t dst_off, size_t bytes) -> hipError_t {
         hipError_t r = hipMemcpyAsync(base + dst_off, stage, bytes, hipMemcpyHostToDevice, c->stream);
         if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
@@ -1466,6 +1492,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
                 geometry(runs[r0 + j], g);
                 d[j] = make_desc(runs[r0 + j], g, r0 + j, c->holes_per_cell);
                 if (runs[r0 + j].sims > 0 && keeps_thresholds(c, d[j])) fixed_thr = 1;
+                if (runs[r0 + j].sims > 0 && uses_window(c, d[j])) window = 1;
             }
             e = upload(sizeof(mcl::RunDesc) * (size_t)r0, sizeof(mcl::RunDesc) * (size_t)k);
         }
@@ -1518,6 +1545,7 @@ int mcl_run_fixed_dt_noalloc(void* ctx, const mcl_run_params* runs, int32_t n_ru
     a.g_cap = g_cap;
     a.wave_search_max = (int32_t)c->wave_search_max;
     a.lookahead_max = (int32_t)c->lookahead_max;
+    a.quiet_window = window;
     a.dump.enabled = 0;
     a.fixed_thr = fixed_thr;
     e = hipEventRecord(c->na_ev0, c->stream);

@@ -26,6 +26,8 @@ struct Ctx {
     double persist_frac = 1.0;     // persistent workgroups per class / device capacity
     double wave_search_max = 32.0; // queue length searched by the whole wave
     double lookahead_max = 8.0;    // live electrons up to which quiet steps are skipped
+    double quiet_window = 1.0;     // 1: after a step with no candidate, the rest of its
+                                   // Philox block is tested at once
     double waves_per_cluster = 0.0; // wavefronts per fixed-dt cluster: 1, 2, 4, 8 or 0 (chosen per launch)
     double mlp_fused = 1.0;        // surrogate inference as one kernel (1) or per layer (0)
     double global_memory = 0.0;    // fixed-dt clusters in the global-memory form: 0 when

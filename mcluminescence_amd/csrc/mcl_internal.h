@@ -65,6 +65,8 @@ struct LaunchArgs {
     int32_t g_cap;          // LDS cell-start capacity (>= G^3 + 1, multiple of 4)
     int32_t wave_search_max;  // queues up to this long are searched one electron per wave (<= 64)
     int32_t lookahead_max;    // with at most this many live electrons, quiet steps are skipped
+    int32_t quiet_window;     // 1: the kernels built with the quiet window (after quiet
+                              // steps, the rest of a Philox block tested at once)
     StateDump dump;
     // global-memory form (clusters beyond one workgroup's LDS): workgroup b keeps its
     // cluster's holes and cell starts (and, unless e_lds, its electron arrays) in
@@ -171,6 +173,17 @@ hipError_t launch_fixed_dt_global(const LaunchArgs& a, int grid, int lds_bytes, 
                                   hipStream_t stream);
 hipError_t prepare_fixed_dt_global(int max_lds);
 hipError_t fixed_dt_global_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
+// The builds with the quiet window (mc_fixed_dt_window.hip,
+// mc_fixed_dt_global_window.hip); the functions above call them when
+// a.quiet_window is set and take both builds into the limits and occupancy.
+hipError_t prepare_fixed_dt_window(int lds_bytes, int max_lds);
+hipError_t launch_fixed_dt_window(const LaunchArgs& a, int grid, int lds_bytes, int waves,
+                                  hipStream_t stream);
+hipError_t fixed_dt_window_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
+hipError_t prepare_fixed_dt_global_window(int max_lds);
+hipError_t launch_fixed_dt_global_window(const LaunchArgs& a, int grid, int lds_bytes, int waves,
+                                         hipStream_t stream);
+hipError_t fixed_dt_global_window_occupancy(int lds_bytes, int waves, int* blocks_per_cu);
 hipError_t launch_event(const EventArgs& a, int lds_bytes, hipStream_t stream);
 // The event kernel's global-memory form: grid workgroups over a.queue's ticket.
 hipError_t launch_event_global(const EventArgs& a, int grid, int lds_bytes, hipStream_t stream);

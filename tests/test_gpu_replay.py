@@ -56,15 +56,17 @@ def test_iso_decay_per_step_bit_exact(gpu_ctx):
     _assert_same(g, gl, o, ol)
 
 
-@pytest.mark.parametrize("fixed_thresholds", [0, 1])
-def test_iso_decay_both_kernel_builds_bit_exact(gpu_ctx, fixed_thresholds):
-    # the isothermal decay on the build that keeps each electron's fire threshold in
-    # its w slot (1) and on the one that computes it from w every step (0); also a
-    # mixed launch (a ramped run beside the isothermal one) and the persistent form
+@pytest.mark.parametrize("fixed_thresholds,quiet_window", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_kernel_builds_bit_exact(gpu_ctx, fixed_thresholds, quiet_window):
+    # each build of the fixed-dt kernels: with or without each electron's fire
+    # threshold kept in its w slot (isothermal runs), with or without the quiet
+    # window (ramped runs); the isothermal decay alone, a mixed launch (a ramped run
+    # beside it) and the persistent graph of the mixed launch
     runs = W.iso_decay_4096x400(sims=128) + W.tl_glow_16x500(sims=16)
     gpu_ctx.tune_set("fixed_thresholds", fixed_thresholds)
+    gpu_ctx.tune_set("quiet_window", quiet_window)
     try:
-        print(f"fixed_thresholds {fixed_thresholds}", flush=True)
+        print(f"fixed_thresholds {fixed_thresholds}, quiet_window {quiet_window}", flush=True)
         g, gl, o, ol, _ = _replay(gpu_ctx, runs[:1])
         _assert_same(g, gl, o, ol)
         g, gl, o, ol, _ = _replay(gpu_ctx, runs)
@@ -74,6 +76,7 @@ def test_iso_decay_both_kernel_builds_bit_exact(gpu_ctx, fixed_thresholds):
         _assert_same(gp, glp, o, ol)
     finally:
         gpu_ctx.tune_set("fixed_thresholds", 1)
+        gpu_ctx.tune_set("quiet_window", 1)
 
 
 def test_tunnelling_per_step_bit_exact(gpu_ctx):
