@@ -653,7 +653,7 @@ def main() -> int:
         "config": {"workload": "iso_decay_500K_4096x400", "clusters": n, "electrons": 400,
                    "holes": 691, "T_K": 500.0, "dt_s": 0.5, "n_steps": 4000, "E_eV": 1.4,
                    "s_per_s": 1e13, "alpha_per_m": 4e9, "rho_per_m3": 1e23,
-                   "kernel": "fixed_dt_kernel<false, true> (1 wave64 per cluster, clusters in LDS, fire thresholds kept in the w slots)", "knobs": knobs},
+                   "kernel": "fixed_dt_kernel<false, true, false> (1 wave64 per cluster, clusters in LDS, fire thresholds kept in the w slots, no quiet window)", "knobs": knobs},
         "work_per_step": {"live_electron_steps": w.live_electron_steps,
                           "distance_evals": w.distance_evals,
                           "recombinations": w.recombinations,
