@@ -148,6 +148,10 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
     int out_base = 0;
     uint32_t o_lum = 0u;
     double o_t = 0.0, o_er = 0.0;
+    // An upper bound of the sum of w over the live electrons that have a hole,
+    // valid until the next recombination changes the electrons or their w.
+    bool w_valid = false;
+    double w_ub = 0.0;
     auto flush_rows = [&](int upto) {
         const int row = out_base + lane;
         if (row < upto) {
@@ -184,27 +188,6 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             const double k = __builtin_fma(ch ? B2 : B1, (double)w, A);
             return p < n_live && (st >> 16) != kNoHole ? k : 0.0;
         };
-        const int rows = (n_live + kWave - 1) / kWave;
-        double K = 0.0;
-        // row totals (the scan's lane-63 values), two rows at a time (independent
-        // chains), added in order
-        int r = 0;
-        for (; r + 1 < rows; r += 2) {
-            const double ra = row_total_d(rate_of(r * kWave + lane));
-            const double rb = row_total_d(rate_of((r + 1) * kWave + lane));
-            if (lane == 0) {
-                rtot[r] = ra;
-                rtot[r + 1] = rb;
-            }
-            K = K + ra;
-            K = K + rb;
-        }
-        if (r < rows) {
-            const double rt = row_total_d(rate_of(r * kWave + lane));
-            if (lane == 0) rtot[r] = rt;
-            K = K + rt;
-        }
-        event_fence<GM>();
         if (i - sb_base >= kWave) {
             const u32x4 o = philox10((uint32_t)(i + lane), 0xFFFFFFFFu, prun, c3f, k0, k1);
             l_uw = o.y;
@@ -214,13 +197,68 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         }
         const int js = i - sb_base;
         const uint32_t u_w = lane_value(l_uw, js);
+        const double e_t = lane_value_d(l_et, js);
         const double lam = (R.D == 0.0 || n_live + filled == N_int)
                                ? 1e-20
                                : (R.D / R.D0) * (double)(N_int - (n_live + filled));
         const double dt_fill =
             fill_never ? __builtin_inf() : (1.0 / lam) * lane_value_d(l_ef, js);
+        // A step whose cap is certainly the minimum needs no K: every k_e is at most
+        // max(B1, B2) w_e + A (rounded up by at most an ulp), so K, summed in
+        // float64 over at most 65535 terms, is below K_ub = (B_max W + n A)(1 +
+        // 2^-30), W an upper bound of the sum of w; then E_t > dt_cap K_ub (1 +
+        // 2^-30) gives E_t / K > dt_cap in float64 as well, and with dt_fill >
+        // dt_cap the step takes dt = dt_cap and no event, as the full step would.
+        bool quiet = false;
+        if (n_live > 0 && dt_fill > dt_cap) {
+            if (!w_valid) {
+                // (any summation order serves a bound: two rows at a time, then the
+                // in-row DPP steps and scalar reads of row_total_d)
+                double ws = 0.0;
+                int p = lane;
+                for (; p + kWave < n_live; p += 2 * kWave) {
+                    const uint32_t sa = est[p], sb2 = est[p + kWave];
+                    const float wa = ew[p], wb = ew[p + kWave];
+                    ws += ((sa >> 16) != kNoHole ? (double)wa : 0.0) +
+                          ((sb2 >> 16) != kNoHole ? (double)wb : 0.0);
+                }
+                if (p < n_live) {
+                    const uint32_t sa = est[p];
+                    const float wa = ew[p];
+                    ws += (sa >> 16) != kNoHole ? (double)wa : 0.0;
+                }
+                w_ub = row_total_d(ws) * (1.0 + 0x1p-30);
+                w_valid = true;
+            }
+            const double bm = two_ch ? fmax(B1, B2) : (always2 ? B2 : B1);
+            const double k_ub = (bm * w_ub + (double)n_live * A) * (1.0 + 0x1p-30);
+            quiet = e_t > dt_cap * k_ub * (1.0 + 0x1p-30);
+        }
+        const int rows = (n_live + kWave - 1) / kWave;
+        double K = 0.0;
+        if (!quiet) {
+            // row totals (the scan's lane-63 values), two rows at a time (independent
+            // chains), added in order
+            int r = 0;
+            for (; r + 1 < rows; r += 2) {
+                const double ra = row_total_d(rate_of(r * kWave + lane));
+                const double rb = row_total_d(rate_of((r + 1) * kWave + lane));
+                if (lane == 0) {
+                    rtot[r] = ra;
+                    rtot[r + 1] = rb;
+                }
+                K = K + ra;
+                K = K + rb;
+            }
+            if (r < rows) {
+                const double rt = row_total_d(rate_of(r * kWave + lane));
+                if (lane == 0) rtot[r] = rt;
+                K = K + rt;
+            }
+            event_fence<GM>();
+        }
         double dt_rec = dt_fill;
-        if (n_live > 0) dt_rec = K > 0.0 ? lane_value_d(l_et, js) / K : __builtin_inf();
+        if (n_live > 0) dt_rec = quiet ? __builtin_inf() : (K > 0.0 ? e_t / K : __builtin_inf());
         const double dt = fmin(fmin(dt_fill, dt_rec), dt_cap);
         t += dt;
         uint8_t ev = 0;
@@ -279,6 +317,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
             }
             n_live = n_new;
             recomb += 1;
+            w_valid = false;
             event_fence<GM>();
             if (nq > 0) {
                 search_queue_nn(grid, egen, est, ew, rq, nq, a.wave_search_max, alpha_f, lane,
