@@ -49,6 +49,21 @@ __device__ __forceinline__ double exp1_of(uint32_t u)
     return -log_own(((double)u + 0.5) * 0x1p-32);
 }
 
+// An upper bound of boltzmann(pre, pre_ok, E, kT) for a step that may skip K:
+// 0 where boltzmann is 0 by its own test; otherwise e^{-E/kT} in float32 (the
+// quotient from a reciprocal and the hardware exp2, exponent clamped at -80 so the
+// result stays a normal float), whose errors, below 1e-4 relative, the factor 1 +
+// 2^-10 covers; +inf (no skip) when pre is not finite and positive.
+__device__ __forceinline__ double boltzmann_ub(double pre, bool pre_ok, double E, double kT)
+{
+    if (!pre_ok) return __builtin_inf();
+    if (kT * 720.0 < E) return 0.0;
+    float x = -(float)E * __builtin_amdgcn_rcpf((float)kT);
+    x = fmaxf(x, -80.0f);
+    const float e = __builtin_amdgcn_exp2f(x * 1.44269504f);
+    return pre * (double)e * (1.0 + 0x1p-10);
+}
+
 // pre e^{-E/kT} as the step computes it (the library's exp of the float64
 // quotient), with the quotient and exp skipped where kT 720 < E: there E / kT >
 // 708, where exp_own returns 0, so the factor is 0 either way (pre_ok: pre finite
@@ -170,9 +185,7 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
         lsteps += (uint64_t)n_live;
         const double T = (R.T_start_C + R.T_rate * t) + 273.15;
         const double kT = R.k_b * T;
-        const double A = boltzmann(R.s, s_ok, R.E_cb, kT);
-        const double B1 = b1_is_a ? A : boltzmann(R.b, b_ok, R.E_loc_1, kT);
-        const double B2 = b2_is_b1 ? B1 : boltzmann(R.b, b_ok, R.E_loc_2, kT);
+        double A = 0.0, B1 = 0.0, B2 = 0.0;   // the exact factors, for a step that forms K
         const uint32_t blk = (uint32_t)i >> 2;
         const int word = i & 3;
         // k_e of slot p, 0 for slots past the live ones and electrons with no hole;
@@ -230,13 +243,20 @@ __device__ __forceinline__ void run_cluster_event(const EventArgs& a, const int 
                 w_ub = row_total_d(ws) * (1.0 + 0x1p-30);
                 w_valid = true;
             }
-            const double bm = two_ch ? fmax(B1, B2) : (always2 ? B2 : B1);
-            const double k_ub = (bm * w_ub + (double)n_live * A) * (1.0 + 0x1p-30);
+            // bounds of the factors (not the factors: a step that skips K needs no more)
+            const double a_ub = boltzmann_ub(R.s, s_ok, R.E_cb, kT);
+            const double b1_ub = b1_is_a ? a_ub : boltzmann_ub(R.b, b_ok, R.E_loc_1, kT);
+            const double b2_ub = b2_is_b1 ? b1_ub : boltzmann_ub(R.b, b_ok, R.E_loc_2, kT);
+            const double bm = two_ch ? fmax(b1_ub, b2_ub) : (always2 ? b2_ub : b1_ub);
+            const double k_ub = (bm * w_ub + (double)n_live * a_ub) * (1.0 + 0x1p-30);
             quiet = e_t > dt_cap * k_ub * (1.0 + 0x1p-30);
         }
         const int rows = (n_live + kWave - 1) / kWave;
         double K = 0.0;
         if (!quiet) {
+            A = boltzmann(R.s, s_ok, R.E_cb, kT);
+            B1 = b1_is_a ? A : boltzmann(R.b, b_ok, R.E_loc_1, kT);
+            B2 = b2_is_b1 ? B1 : boltzmann(R.b, b_ok, R.E_loc_2, kT);
             // row totals (the scan's lane-63 values), two rows at a time (independent
             // chains), added in order
             int r = 0;
