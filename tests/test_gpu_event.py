@@ -35,8 +35,17 @@ def _cases():
     two.Retrap = 0.4
     two = _event(two, 1.0, 3)
     glow = _event(W.tl_glow_16x500(sims=1)[0], 0.1, 4)
+    # every electron in the second channel (Retrap >= 1)
+    ch2 = W.tunnelling_tl12(sims=1, n=150, T_rate=2.0)[0]
+    ch2.E_loc_2 = 1.25
+    ch2.Retrap = 1.0
+    ch2 = _event(ch2, 1.0, 3)
+    # the bench's event workload (0 -> 800 degC at 5 K/s, step cap 0.2 s) on 64 clusters:
+    # most steps take the cap without forming K
+    bench = _event(W.tunnelling_tl12(sims=1, n=400, T_rate=5.0)[0], 1.0, 64)
     return [("tl12_200", tl12, 800.0, 1200), ("iso_400", iso, 2000.0, 500),
-            ("two_channel_150", two, 400.0, 1000), ("glow_500", glow, 400.0, 5000)]
+            ("two_channel_150", two, 400.0, 1000), ("glow_500", glow, 400.0, 5000),
+            ("always_channel_2_150", ch2, 400.0, 1000), ("tl12_5Ks_64x400", bench, 160.0, 4096)]
 
 
 @pytest.mark.parametrize("name,run,duration,max_steps", _cases(), ids=[c[0] for c in _cases()])
