@@ -70,6 +70,24 @@ __device__ __forceinline__ uint64_t lab_nearest(const float4* hole, int n, float
     return wave_min_u64(key);
 }
 
+// The same for every lane's own point at once: each lane scans all n slots (the
+// lanes read the same slot together, one LDS broadcast), the same key per
+// (point, slot) as lab_nearest, so the same minimum.
+__device__ __forceinline__ uint64_t lab_nearest_own(const float4* hole, int n, float x, float y,
+                                                    float z)
+{
+    uint64_t key = ~0ull;
+    for (int s = 0; s < n; ++s) {
+        const float4 h = hole[s];
+        const float dx = h.x - x, dy = h.y - y, dz = h.z - z;
+        const float d2 = h.x == kDeadHoleX ? __builtin_inff()
+                                           : __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const uint64_t k = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)s;
+        key = k < key ? k : key;
+    }
+    return key;
+}
+
 }  // namespace
 
 __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
@@ -106,17 +124,15 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
                               unit_coord(o.z, R.side_nm), 0.0f);
     }
     __syncthreads();
-    // initial nearest holes (Box._rebuild, engine.py:113-119)
-    for (int e = 0; e < n_e; ++e) {
+    // initial nearest holes (Box._rebuild, engine.py:113-119), one electron a lane
+    for (int e = lane; e < n_e; e += kWave) {
         const float4 p = elec[e];
-        const uint64_t key = lab_nearest(hole, n_hs, p.x, p.y, p.z, lane);
-        if (lane == 0) {
-            uint32_t slot;
-            float w;
-            nn_from_key(key, alpha_f, slot, w);
-            enn[e] = slot;
-            elec[e].w = w;
-        }
+        const uint64_t key = lab_nearest_own(hole, n_hs, p.x, p.y, p.z);
+        uint32_t slot;
+        float w;
+        nn_from_key(key, alpha_f, slot, w);
+        enn[e] = slot;
+        elec[e].w = w;
     }
     __syncthreads();
 
@@ -337,20 +353,26 @@ __device__ __forceinline__ void run_cluster_lab(const LabArgs& a, const int c,
             }
             n_e -= 1;
             recomb += 1;
-            for (int e = 0; e < n_e; ++e) {
-                if (enn[e] != hw) continue;            // same value on every lane
-                const float4 p = elec[e];
-                const uint64_t key = lab_nearest(hole, n_hs, p.x, p.y, p.z, lane);
-                __syncthreads();
-                if (lane == 0) {
-                    uint32_t slot;
-                    float w;
-                    nn_from_key(key, alpha_f, slot, w);
-                    enn[e] = slot;
-                    elec[e].w = w;
+            // the electrons whose nearest hole was hw, found row by row by a ballot, each
+            // searched by the whole wave (the holes do not change meanwhile, so the
+            // order of the searches does not matter)
+            for (int r0 = 0; r0 < n_e; r0 += kWave) {
+                uint64_t m = ballot(r0 + lane < n_e && enn[r0 + lane] == hw);
+                while (m != 0) {
+                    const int e = r0 + (int)__builtin_ctzll(m);
+                    m &= m - 1;
+                    const float4 p = elec[e];
+                    const uint64_t key = lab_nearest(hole, n_hs, p.x, p.y, p.z, lane);
+                    if (lane == 0) {
+                        uint32_t slot;
+                        float w;
+                        nn_from_key(key, alpha_f, slot, w);
+                        enn[e] = slot;
+                        elec[e].w = w;
+                    }
                 }
-                __syncthreads();
             }
+            __syncthreads();
             // legacy retrapping: a new pair after the recombination
             if (retrap && (uint64_t)lane_value(l_rw, js) < R.retrap_thr && !add_pair(true)) {
                 status = -2;
