@@ -76,6 +76,24 @@ def test_autotuner_picks_a_measured_point(gpu_ctx, tmp_path):
     gpu_ctx.tune_set("holes_per_cell", 2.0)
 
 
+def test_build_choice_knobs_set_get_and_reject(gpu_ctx):
+    from mcluminescence_amd._abi import MclError
+    for key in ("fixed_thresholds", "quiet_window", "global_memory"):
+        default = gpu_ctx.tune_get(key)
+        try:
+            for v in (0.0, 1.0):
+                gpu_ctx.tune_set(key, v)
+                assert gpu_ctx.tune_get(key) == v
+            for bad in (2.0, -1.0, 0.5):
+                with pytest.raises(MclError):
+                    gpu_ctx.tune_set(key, bad)
+        finally:
+            gpu_ctx.tune_set(key, default)
+    assert gpu_ctx.tune_get("fixed_thresholds") == 1.0
+    assert gpu_ctx.tune_get("quiet_window") == 1.0
+    assert gpu_ctx.tune_get("global_memory") == 0.0
+
+
 def test_reused_device_buffers_give_fresh_context_results(gpu_ctx):
     """Runs take their device buffers from the context's pool: a run that reuses the
     blocks of larger runs (fixed-dt, persistent and event mode) gives exactly what a
